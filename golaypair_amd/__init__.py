@@ -1,0 +1,188 @@
+"""MI355X (gfx950) GolayPair search pipeline.
+
+The compute lives in the C-ABI library ``lib/libgolaypair_gpu.so`` (declared in
+``include/golay_gpu.h``); ``bin/golaypair`` is the drop-in for the reference's
+``main`` command line.  This module is a thin ctypes binding for Python callers
+(tests, bench.py).  There is no CPU fallback: if the library is missing or no
+GPU is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libgolaypair_gpu.so")
+CLI_PATH = os.path.join(PKG_DIR, "bin", "golaypair")
+HEADER_PATH = os.path.join(ROOT_DIR, "include", "golay_gpu.h")
+
+GP_OK = 0
+GP_NO_INPUT = 1
+GP_ERR_ARG = -1
+GP_ERR_HIP = -2
+GP_ERR_CAPACITY = -3
+GP_ERR_UNSUPPORTED = -4
+GP_PSD_SPLIT_BF16 = 0
+GP_PSD_FP32 = 1
+
+_P = ctypes.POINTER
+
+
+class GpCands(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("nkey", ctypes.c_int32), ("count", ctypes.c_int64),
+                ("capacity", ctypes.c_int64), ("words", _P(ctypes.c_int8)), ("keys", _P(ctypes.c_uint8))]
+
+
+class GpPairs(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("count", ctypes.c_int64), ("capacity", ctypes.c_int64),
+                ("a", _P(ctypes.c_int8)), ("b", _P(ctypes.c_int8))]
+
+
+class GpStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in
+                ("words", "psd_decisions", "psd_pass", "band_rechecks", "key_rechecks", "canonical",
+                 "pair_checks", "pairs", "tie_band")] + [("kernel_ms", ctypes.c_double),
+                                                         ("psd_kernel_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GolayError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes); the complete exported surface of golay_gpu.h
+SIGNATURES = {
+    "gp_create": (ctypes.c_int, [_P(ctypes.c_void_p), ctypes.c_int]),
+    "gp_destroy": (None, [ctypes.c_void_p]),
+    "gp_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "gp_set_psd_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gp_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, _P(GpCands), _P(GpCands), _P(GpStats)]),
+    "gp_sort_candidates": (ctypes.c_int, [ctypes.c_void_p, _P(GpCands)]),
+    "gp_match": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(GpCands),
+                                _P(GpCands), _P(GpPairs), _P(GpStats)]),
+    "gp_uncompress_side": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int8), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(GpCands), _P(GpStats)]),
+    "gp_uncompress": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     _P(GpPairs), _P(GpPairs), _P(GpCands), _P(GpCands), _P(GpStats)]),
+    "gp_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(GpPairs), _P(GpPairs),
+                                 _P(GpStats)]),
+    "gp_psd_filter_batch": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int8), ctypes.c_int, _P(ctypes.c_int32),
+                                           _P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_int64,
+                                           _P(ctypes.c_uint8), _P(ctypes.c_float), _P(GpStats)]),
+    "gp_psd_microbench": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, _P(ctypes.c_int32),
+                                         ctypes.c_int, _P(ctypes.c_uint64), _P(ctypes.c_uint64),
+                                         _P(ctypes.c_uint8), ctypes.c_int64, _P(GpStats)]),
+    "gp_microbench_words": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P(ctypes.c_int32), ctypes.c_int,
+                                           ctypes.c_int64, ctypes.c_int64, _P(ctypes.c_int32), _P(ctypes.c_int8),
+                                           ctypes.c_int]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GolayError("libgolaypair_gpu.so not built: run `python -m golaypair_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(_P(ctype))
+
+
+class Context:
+    """One gp_ctx on a HIP device."""
+
+    def __init__(self, device=0, mode=GP_PSD_SPLIT_BF16):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.gp_create(ctypes.byref(h), device)
+        if rc != GP_OK:
+            raise GolayError("gp_create(%d) failed with %d (no GPU?)" % (device, rc))
+        self.h = h
+        self.set_mode(mode)
+
+    def close(self):
+        if self.h:
+            self.lib.gp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, what):
+        if rc < 0:
+            raise GolayError("%s failed (%d): %s" % (what, rc, self.lib.gp_last_error(self.h).decode()))
+        return rc
+
+    def set_mode(self, mode):
+        self.check(self.lib.gp_set_psd_mode(self.h, mode), "gp_set_psd_mode")
+
+    # -- PSD filter -----------------------------------------------------------
+    def psd_filter_batch(self, words, lens, orders, pafs, want_max=False):
+        words = np.ascontiguousarray(words, dtype=np.int8)
+        count, stride = words.shape
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        orders = np.ascontiguousarray(orders, dtype=np.int32)
+        pafs = np.ascontiguousarray(pafs, dtype=np.int32)
+        pas = np.zeros(count, dtype=np.uint8)
+        mx = np.zeros(count, dtype=np.float32) if want_max else None
+        st = GpStats()
+        rc = self.lib.gp_psd_filter_batch(self.h, ptr(words, ctypes.c_int8), stride, ptr(lens, ctypes.c_int32),
+                                          ptr(orders, ctypes.c_int32), ptr(pafs, ctypes.c_int32), count,
+                                          ptr(pas, ctypes.c_uint8),
+                                          ptr(mx, ctypes.c_float) if want_max else None, ctypes.byref(st))
+        self.check(rc, "gp_psd_filter_batch")
+        return pas, mx, st
+
+    def psd_microbench(self, seed, mix, pass_cap=0):
+        mix = np.ascontiguousarray(mix, dtype=np.int32).reshape(-1, 5)
+        count = int(mix[:, 4].sum())
+        acc = np.zeros(len(mix), dtype=np.uint64)
+        dig = ctypes.c_uint64()
+        pas = np.zeros(max(pass_cap, 1), dtype=np.uint8)
+        st = GpStats()
+        rc = self.lib.gp_psd_microbench(self.h, seed, count, ptr(mix, ctypes.c_int32), len(mix),
+                                        ptr(acc, ctypes.c_uint64), ctypes.byref(dig),
+                                        ptr(pas, ctypes.c_uint8) if pass_cap else None, pass_cap,
+                                        ctypes.byref(st))
+        self.check(rc, "gp_psd_microbench")
+        return acc, dig.value, pas[:pass_cap], st
+
+
+def microbench_words(seed, mix, first, rows, stride=64):
+    """Host copy of the microbenchmark generator: (kinds, words) of candidates
+    first..first+rows-1 (words zero padded to `stride`)."""
+    lib = load_library()
+    mix = np.ascontiguousarray(mix, dtype=np.int32).reshape(-1, 5)
+    count = int(mix[:, 4].sum())
+    kinds = np.zeros(rows, dtype=np.int32)
+    out = np.zeros((rows, stride), dtype=np.int8)
+    rc = lib.gp_microbench_words(seed, count, ptr(mix, ctypes.c_int32), len(mix), first, rows,
+                                 ptr(kinds, ctypes.c_int32), ptr(out, ctypes.c_int8), stride)
+    if rc != GP_OK:
+        raise GolayError("gp_microbench_words failed")
+    return kinds, out

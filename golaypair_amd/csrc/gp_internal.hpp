@@ -1,0 +1,123 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gp_common.hpp"
+#include "golay_gpu.h"
+
+namespace gp {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GP_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            throw ::gp::HipError(std::string(#expr) + ": " + hipGetErrorString(_e) + " at " +   \
+                                 __FILE__ + ":" + std::to_string(__LINE__));                      \
+    } while (0)
+
+// Growable device allocation owned by the context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void* ensure(size_t n) {
+        if (n <= bytes) return p;
+        if (p) { GP_HIP(hipFree(p)); p = nullptr; bytes = 0; }
+        size_t want = n < 256 ? 256 : n;
+        GP_HIP(hipMalloc(&p, want));
+        bytes = want;
+        return p;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// One PSD table per word length: MFMA A fragments, FP32 and FP64 twiddles.
+struct PsdTables {
+    std::vector<PsdTableDesc> desc;       // index = table id
+    std::map<int, int> by_n;              // n -> table id
+    std::vector<uint4> frags;             // all tables' A fragments
+    std::vector<float> trig32;            // per table: cos[n], sin[n]
+    std::vector<double> trig64;           // per table: cos[n], sin[n]
+    DevBuf d_desc, d_frags, d_trig32, d_trig64;
+    bool dirty = true;
+    int get(int n);                       // builds the table on first use
+    void upload(hipStream_t s);
+};
+
+// Per-table filter thresholds for one call: accept if max < lo, reject if
+// max > hi, FP64 recheck otherwise; thr = 2*order - paf + 0.001.
+struct FilterThr {
+    float lo, hi;
+    double thr;
+};
+
+struct TileDesc {
+    int32_t table;
+    int32_t rows;
+    int64_t first_row;
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    void init() { GP_HIP(hipEventCreate(&a)); GP_HIP(hipEventCreate(&b)); }
+    ~Timer() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); }
+};
+
+}  // namespace gp
+
+struct gp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int psd_mode = GP_PSD_SPLIT_BF16;
+    std::string err;
+    gp::PsdTables tables;
+    // scratch owned by the context
+    gp::DevBuf words, tiles, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    int num_cus = 256;
+};
+
+namespace gp {
+
+// Error bound on Re/Im of the split-BF16 contraction for words with
+// sum |x| <= s1 using table `d`.
+double split_bf16_E(const PsdTableDesc& d, double s1);
+// Same for the plain FP32 direct DFT.
+double fp32_E(int n, double s1);
+// Thresholds for `n` with threshold thr, error bound E.
+FilterThr make_thr(double thr, double E);
+
+// Filters rows [0, nrows) of `d_words` (stride bytes per row, rows zero
+// padded to 16*ceil(n/16)).  `d_tiles` lists 32-row tiles of one table each
+// (split mode) and `segs` the same rows as per-table segments (FP32 mode).
+// Writes decisions (kReject/kAccept) after the FP64 recheck of band rows.
+// Returns PSD-kernel device time in ms; stats counts band rechecks and ties.
+struct BatchSeg {
+    int32_t table;
+    int32_t pad;
+    int64_t first_row;
+    int64_t rows;
+};
+double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nrows,
+                      const std::vector<BatchSeg>& segs, const std::vector<FilterThr>& thr_by_table,
+                      uint8_t* d_decision, gp_stats* stats, float* d_maxpsd = nullptr);
+
+}  // namespace gp

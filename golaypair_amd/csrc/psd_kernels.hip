@@ -1,0 +1,348 @@
+// Batched PSD filter kernels (gfx950):
+//   k_psd_mfma    split-BF16 MFMA contraction, 32 candidates per tile
+//   k_psd_fp32    plain FP32 direct DFT, one candidate per lane
+//   k_psd_fp64    FP64 direct DFT for candidates inside the error band
+// Semantics: src/utils/fourier.cpp:25-52 (reject iff PSD_k > 2*order - paf +
+// 0.001 for some 1 <= k <= n/2).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "gp_internal.hpp"
+#include "psd_core.hpp"
+
+namespace gp {
+
+// ---------------------------------------------------------------------------
+// host: tables
+
+namespace {
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+double bf16_to_double(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+}  // namespace
+
+int PsdTables::get(int n) {
+    auto it = by_n.find(n);
+    if (it != by_n.end()) return it->second;
+    if (n < 2 || n > kMaxN) throw ArgError("word length " + std::to_string(n) + " outside [2, 64]");
+    PsdTableDesc d{};
+    d.n = n;
+    d.K = n / 2;
+    d.nb = (n + 15) / 16;
+    d.mt = (d.K + 15) / 16;
+    // FP64 twiddles, symmetric in m <-> n-m (psd[k] == psd[n-k] bit-exactly).
+    std::vector<double> c(n), s(n);
+    for (int m = 0; 2 * m <= n; ++m) {
+        double a = 2.0 * M_PI * (double)m / (double)n;
+        double cm = std::cos(a), sm = std::sin(a);
+        if (m == 0) { cm = 1.0; sm = 0.0; }
+        if (2 * m == n) { cm = -1.0; sm = 0.0; }
+        c[m] = cm; s[m] = sm;
+        if (m != 0 && 2 * m != n) { c[n - m] = cm; s[n - m] = -sm; }
+    }
+    d.trig_off = (int32_t)trig64.size();
+    for (int m = 0; m < n; ++m) trig64.push_back(c[m]);
+    for (int m = 0; m < n; ++m) trig64.push_back(s[m]);
+    for (int m = 0; m < n; ++m) trig32.push_back((float)c[m]);
+    for (int m = 0; m < n; ++m) trig32.push_back((float)s[m]);
+    // A fragments.
+    d.frag_off = (int32_t)frags.size();
+    double rho = 0.0;
+    for (int t = 0; t < d.mt; ++t)
+        for (int st = 0; st < 2 * d.nb; ++st)
+            for (int lane = 0; lane < 64; ++lane) {
+                uint16_t h[8];
+                for (int i = 0; i < 8; ++i) {
+                    int row = lane % 32;
+                    int e = 16 * (st % d.nb) + 8 * (lane / 32) + i;
+                    int part = st / d.nb;
+                    int k = 16 * t + row % 16 + 1;
+                    bool im = row >= 16;
+                    uint16_t v = 0;
+                    if (e < n && k <= d.K) {
+                        int m = (int)(((long)k * e) % n);
+                        double w = im ? s[m] : c[m];
+                        uint16_t hi = bf16_rne((float)w);
+                        double r1 = w - bf16_to_double(hi);
+                        uint16_t lo = bf16_rne((float)r1);
+                        double r2 = std::fabs(r1 - bf16_to_double(lo));
+                        rho = std::max(rho, r2);
+                        v = part ? lo : hi;
+                    }
+                    h[i] = v;
+                }
+                uint4 q;
+                q.x = h[0] | ((uint32_t)h[1] << 16);
+                q.y = h[2] | ((uint32_t)h[3] << 16);
+                q.z = h[4] | ((uint32_t)h[5] << 16);
+                q.w = h[6] | ((uint32_t)h[7] << 16);
+                frags.push_back(q);
+            }
+    d.rho_max = (float)(rho * 1.0000001);
+    int id = (int)desc.size();
+    desc.push_back(d);
+    by_n[n] = id;
+    dirty = true;
+    return id;
+}
+
+void PsdTables::upload(hipStream_t st) {
+    if (!dirty) return;
+    d_desc.ensure(desc.size() * sizeof(PsdTableDesc));
+    d_frags.ensure(frags.size() * sizeof(uint4));
+    d_trig32.ensure(trig32.size() * sizeof(float));
+    d_trig64.ensure(trig64.size() * sizeof(double));
+    GP_HIP(hipMemcpyAsync(d_desc.p, desc.data(), desc.size() * sizeof(PsdTableDesc), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_frags.p, frags.data(), frags.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_trig32.p, trig32.data(), trig32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_trig64.p, trig64.data(), trig64.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    GP_HIP(hipStreamSynchronize(st));
+    dirty = false;
+}
+
+double split_bf16_E(const PsdTableDesc& d, double s1) {
+    // representation error of W (rho) + FP32 accumulation over the 2*16*nb
+    // exact products (+1 for the initial accumulator), each add taken as a
+    // truncation (2^-23 relative of a partial sum bounded by sum |products|).
+    const double adds = 32.0 * d.nb + 1.0;
+    return s1 * ((double)d.rho_max + 1.004 * adds * 0x1p-23);
+}
+
+double fp32_E(int n, double s1) {
+    // FP32 twiddles (2^-24 relative) and n FMAs per sum (2^-24 each, on partial
+    // sums bounded by s1).
+    return s1 * ((double)(n + 2) * 0x1p-24) * 1.01;
+}
+
+FilterThr make_thr(double thr, double E) {
+    const double eps = psd_eps(E, thr);
+    FilterThr t;
+    t.thr = thr;
+    t.hi = std::nextafter((float)(thr + eps), INFINITY);
+    t.lo = std::nextafter((float)(thr - eps), -INFINITY);
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// device
+
+struct BandCtl {
+    unsigned int count;   // band rows appended
+    unsigned int ties;    // FP64 values within 1e-9 of the threshold
+};
+
+template <int NB, int MT>
+__device__ __forceinline__ void psd_tile(const int8_t* __restrict__ words, int stride, const TileDesc td,
+                                         const uint4* __restrict__ afrag, const FilterThr th,
+                                         uint8_t* __restrict__ decision, BandCtl* ctl,
+                                         int64_t* __restrict__ band_rows, float* __restrict__ maxpsd, int lane) {
+    const int col = lane & 31, half = lane >> 5;
+    const bool valid = col < td.rows;
+    uint4 bop[NB];
+    const int8_t* w = words + (td.first_row + col) * (int64_t)stride + 8 * half;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (valid) {
+            uint2 v = *reinterpret_cast<const uint2*>(w + 16 * b);
+            bop[b] = pack8_bf16(v.x, v.y);
+        } else {
+            bop[b] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    f32x16 acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = f32x16{};
+    mfma_tile<NB, MT>(afrag, bop, acc, lane);
+    const float m = tile_max_psd<MT>(acc);
+    if (half == 0 && valid) {
+        uint8_t d = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
+        const int64_t row = td.first_row + col;
+        decision[row] = d;
+        if (maxpsd) maxpsd[row] = m;
+        if (d == kBand) {
+            unsigned int i = atomicAdd(&ctl->count, 1u);
+            band_rows[i] = row;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ words, int stride,
+                                                  const TileDesc* __restrict__ tiles, int64_t ntiles,
+                                                  const PsdTableDesc* __restrict__ tabs,
+                                                  const uint4* __restrict__ frags,
+                                                  const FilterThr* __restrict__ thr,
+                                                  uint8_t* __restrict__ decision, BandCtl* ctl,
+                                                  int64_t* __restrict__ band_rows, float* __restrict__ maxpsd) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t ti = wave; ti < ntiles; ti += nwaves) {
+        const TileDesc td = tiles[ti];
+        const PsdTableDesc d = tabs[td.table];
+        const uint4* af = frags + d.frag_off;
+        const FilterThr th = thr[td.table];
+        if (d.nb == 1) psd_tile<1, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+        else if (d.nb == 2 && d.mt == 1) psd_tile<2, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+        else if (d.nb == 3 && d.mt == 1) psd_tile<3, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+        else if (d.nb == 3) psd_tile<3, 2>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+        else psd_tile<4, 2>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+    }
+}
+
+// Plain FP32 direct DFT: one candidate per lane, rows in per-table segments.
+template <int KMAX>
+__device__ __forceinline__ float fp32_max_psd(const int8_t* __restrict__ w, int n, int K,
+                                              const float* __restrict__ c, const float* __restrict__ s) {
+    float re[KMAX], im[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { re[k] = 0.f; im[k] = 0.f; }
+    for (int p = 0; p < n; ++p) {
+        const float x = (float)w[p];
+        int m = 0;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            m += p;
+            if (m >= n) m -= n;
+            re[k] = __builtin_fmaf(x, c[m], re[k]);
+            im[k] = __builtin_fmaf(x, s[m], im[k]);
+        }
+    }
+    float mx = 0.f;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+        if (k < K) mx = fmaxf(mx, __builtin_fmaf(re[k], re[k], im[k] * im[k]));
+    return mx;
+}
+
+__global__ void __launch_bounds__(256) k_psd_fp32(const int8_t* __restrict__ words, int stride,
+                                                  const BatchSeg* __restrict__ segs, int nseg,
+                                                  const PsdTableDesc* __restrict__ tabs,
+                                                  const float* __restrict__ trig32,
+                                                  const FilterThr* __restrict__ thr,
+                                                  uint8_t* __restrict__ decision, BandCtl* ctl,
+                                                  int64_t* __restrict__ band_rows, float* __restrict__ maxpsd) {
+    __shared__ float lds[8][2 * kMaxN];
+    __shared__ int seg_table[8];
+    if (threadIdx.x < 8 && (int)threadIdx.x < nseg) seg_table[threadIdx.x] = segs[threadIdx.x].table;
+    __syncthreads();
+    for (int sg = 0; sg < nseg && sg < 8; ++sg) {
+        const PsdTableDesc d = tabs[seg_table[sg]];
+        for (int i = threadIdx.x; i < 2 * d.n; i += blockDim.x) lds[sg][i] = trig32[d.trig_off + i];
+    }
+    __syncthreads();
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gsz = (int64_t)gridDim.x * blockDim.x;
+    for (int sg = 0; sg < nseg && sg < 8; ++sg) {
+        const BatchSeg S = segs[sg];
+        const PsdTableDesc d = tabs[S.table];
+        const FilterThr th = thr[S.table];
+        const float* c = lds[sg];
+        const float* s = lds[sg] + d.n;
+        for (int64_t r = gid; r < S.rows; r += gsz) {
+            const int64_t row = S.first_row + r;
+            const int8_t* w = words + row * (int64_t)stride;
+            float m = d.K <= 16 ? fp32_max_psd<16>(w, d.n, d.K, c, s) : fp32_max_psd<32>(w, d.n, d.K, c, s);
+            uint8_t dec = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
+            decision[row] = dec;
+            if (maxpsd) maxpsd[row] = m;
+            if (dec == kBand) {
+                unsigned int i = atomicAdd(&ctl->count, 1u);
+                band_rows[i] = row;
+            }
+        }
+    }
+}
+
+// FP64 recheck of band rows (row -> table via segments).
+__global__ void k_psd_fp64(const int8_t* __restrict__ words, int stride, const BatchSeg* __restrict__ segs,
+                           int nseg, const PsdTableDesc* __restrict__ tabs, const double* __restrict__ trig64,
+                           const FilterThr* __restrict__ thr, uint8_t* __restrict__ decision, BandCtl* ctl,
+                           const int64_t* __restrict__ band_rows) {
+    const unsigned int nb = ctl->count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+        const int64_t row = band_rows[i];
+        int sg = 0;
+        while (sg + 1 < nseg && row >= segs[sg + 1].first_row) ++sg;
+        const PsdTableDesc d = tabs[segs[sg].table];
+        const double* c = trig64 + d.trig_off;
+        const double* s = c + d.n;
+        const double m = fp64_max_psd(words + row * (int64_t)stride, d.n, c, s);
+        const double t = thr[segs[sg].table].thr;
+        if (fabs(m - t) < 1e-9) atomicAdd(&ctl->ties, 1u);
+        decision[row] = m > t ? kReject : kAccept;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+
+double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nrows,
+                      const std::vector<BatchSeg>& segs, const std::vector<FilterThr>& thr_by_table,
+                      uint8_t* d_decision, gp_stats* stats, float* d_maxpsd) {
+    if (segs.size() > 8) throw ArgError("at most 8 word lengths per PSD batch");
+    hipStream_t st = ctx->stream;
+    ctx->tables.upload(st);
+    // tiles
+    std::vector<TileDesc> tiles;
+    for (const BatchSeg& s : segs)
+        for (int64_t r = 0; r < s.rows; r += 32)
+            tiles.push_back(TileDesc{s.table, (int32_t)std::min<int64_t>(32, s.rows - r), s.first_row + r});
+    BandCtl* ctl = static_cast<BandCtl*>(ctx->counters.ensure(sizeof(BandCtl)));
+    int64_t* band = static_cast<int64_t*>(ctx->band_rows.ensure(std::max<int64_t>(nrows, 1) * sizeof(int64_t)));
+    TileDesc* d_tiles = static_cast<TileDesc*>(ctx->tiles.ensure(std::max<size_t>(tiles.size(), 1) * sizeof(TileDesc)));
+    FilterThr* d_thr = static_cast<FilterThr*>(ctx->thr.ensure(thr_by_table.size() * sizeof(FilterThr) + 64));
+    BatchSeg* d_segs = static_cast<BatchSeg*>(ctx->aux5.ensure(segs.size() * sizeof(BatchSeg) + 64));
+    GP_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_thr, thr_by_table.data(), thr_by_table.size() * sizeof(FilterThr), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemsetAsync(ctl, 0, sizeof(BandCtl), st));
+    GP_HIP(hipEventRecord(ctx->ev0, st));
+    if (ctx->psd_mode == GP_PSD_SPLIT_BF16) {
+        int64_t waves = std::min<int64_t>((int64_t)tiles.size(), (int64_t)ctx->num_cus * 16);
+        int blocks = (int)std::max<int64_t>(1, (waves + 3) / 4);
+        if (!tiles.empty())
+            hipLaunchKernelGGL(k_psd_mfma, dim3(blocks), dim3(256), 0, st, d_words, stride, d_tiles,
+                               (int64_t)tiles.size(), ctx->tables.d_desc.as<PsdTableDesc>(),
+                               ctx->tables.d_frags.as<uint4>(), d_thr, d_decision, ctl, band, d_maxpsd);
+    } else {
+        int blocks = (int)std::min<int64_t>((nrows + 255) / 256, (int64_t)ctx->num_cus * 8);
+        if (nrows > 0)
+            hipLaunchKernelGGL(k_psd_fp32, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_words, stride, d_segs,
+                               (int)segs.size(), ctx->tables.d_desc.as<PsdTableDesc>(),
+                               ctx->tables.d_trig32.as<float>(), d_thr, d_decision, ctl, band, d_maxpsd);
+    }
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipEventRecord(ctx->ev1, st));
+    hipLaunchKernelGGL(k_psd_fp64, dim3(ctx->num_cus), dim3(256), 0, st, d_words, stride, d_segs, (int)segs.size(),
+                       ctx->tables.d_desc.as<PsdTableDesc>(), ctx->tables.d_trig64.as<double>(), d_thr, d_decision,
+                       ctl, band);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipEventRecord(ctx->ev2, st));
+    BandCtl h{};
+    GP_HIP(hipMemcpyAsync(&h, ctl, sizeof(BandCtl), hipMemcpyDeviceToHost, st));
+    GP_HIP(hipStreamSynchronize(st));
+    float ms_main = 0, ms_band = 0;
+    GP_HIP(hipEventElapsedTime(&ms_main, ctx->ev0, ctx->ev1));
+    GP_HIP(hipEventElapsedTime(&ms_band, ctx->ev1, ctx->ev2));
+    if (stats) {
+        stats->band_rechecks += h.count;
+        stats->tie_band += h.ties;
+        for (const BatchSeg& sg : segs) stats->psd_decisions += (uint64_t)sg.rows;
+        stats->psd_kernel_ms += ms_main + ms_band;
+        stats->kernel_ms += ms_main + ms_band;
+    }
+    return ms_main;
+}
+
+}  // namespace gp
