@@ -78,6 +78,8 @@ SIGNATURES = {
     "gp_psd_microbench": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, _P(ctypes.c_int32),
                                          ctypes.c_int, _P(ctypes.c_uint64), _P(ctypes.c_uint64),
                                          _P(ctypes.c_uint8), ctypes.c_int64, _P(GpStats)]),
+    "gp_psd_keys": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int8), ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                   ctypes.c_int, ctypes.c_int, _P(ctypes.c_uint8), _P(GpStats)]),
     "gp_microbench_words": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P(ctypes.c_int32), ctypes.c_int,
                                            ctypes.c_int64, ctypes.c_int64, _P(ctypes.c_int32), _P(ctypes.c_int8),
                                            ctypes.c_int]),
@@ -171,6 +173,102 @@ class Context:
                                         ctypes.byref(st))
         self.check(rc, "gp_psd_microbench")
         return acc, dig.value, pas[:pass_cap], st
+
+
+class Cands:
+    """Host candidate set: words (m, n) int8 and key digits (m, n//2) uint8."""
+
+    def __init__(self, words, keys):
+        self.words = np.ascontiguousarray(words, dtype=np.int8)
+        self.keys = np.ascontiguousarray(keys, dtype=np.uint8)
+
+    @property
+    def n(self):
+        return self.words.shape[1]
+
+    def __len__(self):
+        return self.words.shape[0]
+
+    def struct(self):
+        m, n = self.words.shape
+        c = GpCands(n, n // 2, m, m, ptr(self.words, ctypes.c_int8), ptr(self.keys, ctypes.c_uint8))
+        return c
+
+    def lines(self):
+        return [cand_line(w, k) for w, k in zip(self.words, self.keys)]
+
+
+def cand_line(word, key):
+    """Candidate line text without newline (io.cpp:14-36)."""
+    n = len(word)
+    K = n // 2
+    digits = "".join(str(int(key[(k if k <= K else n - k) - 1])) for k in range(1, n))
+    return "PSD" + digits + " " + "".join("%d " % v for v in word)
+
+
+def pair_line(a, b):
+    """Pair line text without newline (io.cpp:75-95)."""
+    return "".join("%d " % v for v in a) + " " + "".join("%d " % v for v in b)
+
+
+def _empty_cands(n, cap):
+    return Cands(np.zeros((cap, n), np.int8), np.zeros((cap, n // 2), np.uint8))
+
+
+def _ctx_methods():
+    def generate(self, order, compress, paf=0, job=0, njobs=1):
+        n = order // compress
+        cap = 1 << 16
+        while True:
+            a, b = _empty_cands(n, cap), _empty_cands(n, cap)
+            sa, sb = a.struct(), b.struct()
+            st = GpStats()
+            rc = self.lib.gp_generate(self.h, order, compress, paf, job, njobs, ctypes.byref(sa), ctypes.byref(sb),
+                                      ctypes.byref(st))
+            if rc == GP_ERR_CAPACITY:
+                cap = max(sa.count, sb.count) + 1
+                continue
+            self.check(rc, "gp_generate")
+            if rc == GP_NO_INPUT:
+                return None, None, st
+            return Cands(a.words[:sa.count], a.keys[:sa.count]), Cands(b.words[:sb.count], b.keys[:sb.count]), st
+
+    def sort_candidates(self, c):
+        c = Cands(c.words.copy(), c.keys.copy())
+        s = c.struct()
+        self.check(self.lib.gp_sort_candidates(self.h, ctypes.byref(s)), "gp_sort_candidates")
+        return Cands(c.words[:s.count], c.keys[:s.count])
+
+    def match(self, order, compress, paf, a, b):
+        n = order // compress
+        cap = max(1024, 4 * (len(a) + len(b)))
+        while True:
+            pa = np.zeros((cap, n), np.int8)
+            pb = np.zeros((cap, n), np.int8)
+            out = GpPairs(n, 0, cap, ptr(pa, ctypes.c_int8), ptr(pb, ctypes.c_int8))
+            sa, sb = a.struct(), b.struct()
+            st = GpStats()
+            rc = self.lib.gp_match(self.h, order, compress, paf, ctypes.byref(sa), ctypes.byref(sb),
+                                   ctypes.byref(out), ctypes.byref(st))
+            if rc == GP_ERR_CAPACITY:
+                cap = out.count + 1
+                continue
+            self.check(rc, "gp_match")
+            return rc, pa[:out.count], pb[:out.count], st
+
+    def psd_keys(self, words, bound, inverted):
+        words = np.ascontiguousarray(words, dtype=np.int8)
+        m, n = words.shape
+        keys = np.zeros((m, n // 2), np.uint8)
+        self.check(self.lib.gp_psd_keys(self.h, ptr(words, ctypes.c_int8), n, n, m, bound, int(inverted),
+                                        ptr(keys, ctypes.c_uint8), None), "gp_psd_keys")
+        return keys
+
+    for f in (generate, sort_candidates, match, psd_keys):
+        setattr(Context, f.__name__, f)
+
+
+_ctx_methods()
 
 
 def microbench_words(seed, mix, first, rows, stride=64):

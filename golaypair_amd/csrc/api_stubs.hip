@@ -2,18 +2,6 @@
 #include "gp_internal.hpp"
 
 extern "C" {
-int gp_generate(gp_ctx* ctx, int, int, int, int, int, gp_cands*, gp_cands*, gp_stats*) {
-    if (ctx) ctx->err = "gp_generate: not implemented";
-    return GP_ERR_UNSUPPORTED;
-}
-int gp_sort_candidates(gp_ctx* ctx, gp_cands*) {
-    if (ctx) ctx->err = "gp_sort_candidates: not implemented";
-    return GP_ERR_UNSUPPORTED;
-}
-int gp_match(gp_ctx* ctx, int, int, int, const gp_cands*, const gp_cands*, gp_pairs*, gp_stats*) {
-    if (ctx) ctx->err = "gp_match: not implemented";
-    return GP_ERR_UNSUPPORTED;
-}
 int gp_uncompress_side(gp_ctx* ctx, int8_t*, int, int, int, int, int, gp_cands*, gp_stats*) {
     if (ctx) ctx->err = "gp_uncompress_side: not implemented";
     return GP_ERR_UNSUPPORTED;

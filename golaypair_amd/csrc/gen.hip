@@ -1,0 +1,427 @@
+// Candidate generation (replaces generate_hybrid, src/core/generate/generate_hybrid.cpp:107-163).
+//
+// The reference walks all necklaces in FKM order (lyndon.cpp:47-121) and keeps
+// a necklace w when rowsum(w) matches a decomposition, the PSD filter passes and
+// w is lexicographically smallest among its images under the index group
+// (orderly_equivalence.cpp:53-68): AGL(1,n) = {i -> u i + t} for A, the dihedral
+// group {i -> +-i + t} for B.  Lex-min under a group containing the rotations
+// implies a necklace, so the kept set is
+//   {w : rowsum(w) = x, w <= w o g for all g in G, PSD filter passes}.
+// Here that set is enumerated directly: words of fixed letter-index sum are
+// ranked in lexicographic order, each wavefront takes one contiguous rank range
+// (one contiguous sub-range per lane, successor steps in between), rejects
+// non-canonical words in registers, and appends the survivors; the PSD filter
+// (split-BF16 MFMA + FP64 band) and FP64 keys follow on the survivors.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <set>
+
+#include "gp_internal.hpp"
+#include "gp_host.hpp"
+#include "psd_core.hpp"
+
+namespace gp {
+
+typedef unsigned __int128 u128;
+
+struct GenSeg {       // one letter-index sum J
+    int32_t J;
+    int32_t side;     // bit 0: A candidates (AGL), bit 1: B candidates (dihedral)
+    int64_t count;    // words with index sum J
+};
+struct GenTask {
+    int32_t seg;
+    int32_t per_lane; // ranks per lane
+    int64_t start;    // first rank of the wave
+    int64_t end;      // one past the last rank of the segment range
+};
+struct GenParams {
+    int32_t n, k;
+    int32_t nunits_a;          // multipliers u != 1 for A (units of n)
+    int32_t nunits_b;          // multipliers u != 1 for B (n-1 when n > 2)
+    int32_t units_a[64];
+    int32_t units_b[2];
+    int32_t shard_len;         // prefix length of the job split (0 = none)
+    int32_t pad;
+    int64_t capacity;          // survivor slots
+};
+struct GenCtl {
+    unsigned long long survivors;
+    unsigned long long words;
+};
+
+__device__ __forceinline__ int digit_at(u128 w, int n, int p) { return (int)((w >> (4 * (n - 1 - p))) & 15); }
+
+__device__ __forceinline__ u128 mask_n(int n) {
+    return n >= 32 ? ~(u128)0 : (((u128)1 << (4 * n)) - 1);
+}
+
+__device__ __forceinline__ u128 rot(u128 w, int n, int t, u128 mask) {
+    if (t == 0) return w;
+    return ((w << (4 * t)) | (w >> (4 * (n - t)))) & mask;
+}
+
+// w <= every rotation of v (v may equal w).
+__device__ __forceinline__ bool le_all_rotations(u128 w, u128 v, int n, u128 mask, int t0) {
+    for (int t = t0; t < n; ++t)
+        if (rot(v, n, t, mask) < w) return false;
+    return true;
+}
+
+__device__ __forceinline__ u128 decimate(u128 w, int n, int u) {
+    u128 r = 0;
+    int idx = 0;
+    for (int i = 0; i < n; ++i) {
+        r = (r << 4) | (u128)digit_at(w, n, idx);
+        idx += u;
+        if (idx >= n) idx -= n;
+    }
+    return r;
+}
+
+// Unrank rank r of the words with index sum J (lexicographic order).
+__device__ u128 unrank(int64_t r, int J, int n, int k, const uint64_t* __restrict__ cnt, int cstride) {
+    u128 w = 0;
+    for (int p = 0; p < n; ++p) {
+        const int rem = n - 1 - p;
+        int d = 0;
+        for (; d < k; ++d) {
+            if (J - d < 0) break;
+            const int64_t c = (int64_t)cnt[rem * cstride + (J - d)];
+            if (r < c) break;
+            r -= c;
+        }
+        w = (w << 4) | (u128)d;
+        J -= d;
+    }
+    return w;
+}
+
+// Next word with the same digit sum in lexicographic order (false at the end).
+__device__ bool successor(u128& w, int n, int k) {
+    int S = 0;
+    for (int p = n - 2; p >= 0; --p) {
+        S += digit_at(w, n, p + 1);
+        const int d = digit_at(w, n, p);
+        if (d < k - 1 && S >= 1) {
+            // digits 0..p-1 unchanged, digit p + 1, suffix = smallest arrangement of S - 1
+            u128 head = w >> (4 * (n - p));          // digits 0..p-1
+            u128 out = (head << 4) | (u128)(d + 1);
+            int rem = S - 1;
+            const int L = n - 1 - p;
+            u128 suf = 0;
+            int full = rem / (k - 1);
+            int part = rem - full * (k - 1);
+            // suffix (left to right): zeros, then `part` (if full < L), then full x (k-1)
+            for (int q = 0; q < L; ++q) {
+                int v;
+                const int from_right = L - 1 - q;
+                if (from_right < full) v = k - 1;
+                else if (from_right == full) v = part;
+                else v = 0;
+                suf = (suf << 4) | (u128)v;
+            }
+            w = (out << (4 * L)) | suf;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __restrict__ segs,
+                                                  const GenTask* __restrict__ tasks, int64_t ntasks,
+                                                  const uint64_t* __restrict__ cnt, int cstride,
+                                                  const uint32_t* __restrict__ shard_bits,
+                                                  u128* __restrict__ surv_w, uint8_t* __restrict__ surv_side,
+                                                  GenCtl* ctl) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int n = P.n, k = P.k;
+    const u128 mask = mask_n(n);
+    unsigned long long nwords = 0;
+    for (int64_t ti = wave; ti < ntasks; ti += nwaves) {
+        const GenTask T = tasks[ti];
+        const GenSeg S = segs[T.seg];
+        int64_t r0 = T.start + (int64_t)lane * T.per_lane;
+        int64_t r1 = min(r0 + (int64_t)T.per_lane, T.end);
+        u128 w = 0;
+        bool live = r0 < r1;
+        if (live) w = unrank(r0, S.J, n, k, cnt, cstride);
+        int64_t r = r0;
+        // every lane steps until the whole wave is done
+        while (__any(live)) {
+            bool keepA = false, keepB = false;
+            if (live) {
+                ++nwords;
+                bool ok = true;
+                if (P.shard_len > 0) {
+                    uint32_t idx = 0;
+                    for (int p = 0; p < P.shard_len; ++p) idx = idx * (uint32_t)k + (uint32_t)digit_at(w, n, p);
+                    ok = (shard_bits[idx >> 5] >> (idx & 31)) & 1u;
+                }
+                // rotations (u = 1) first: the necklace condition
+                if (ok) ok = le_all_rotations(w, w, n, mask, 1);
+                if (ok) {
+                    bool dih = true;
+                    for (int j = 0; j < P.nunits_b && dih; ++j)
+                        dih = le_all_rotations(w, decimate(w, n, P.units_b[j]), n, mask, 0);
+                    keepB = dih && (S.side & 2);
+                    if (dih && (S.side & 1)) {
+                        bool agl = true;
+                        for (int j = 0; j < P.nunits_a && agl; ++j) {
+                            const int u = P.units_a[j];
+                            if (u == n - 1) continue;  // covered by the dihedral test
+                            agl = le_all_rotations(w, decimate(w, n, u), n, mask, 0);
+                        }
+                        keepA = agl;
+                    }
+                }
+            }
+            const unsigned long long ba = __ballot(keepA), bb = __ballot(keepB);
+            const int tot = __popcll(ba) + __popcll(bb);
+            if (tot) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&ctl->survivors, (unsigned long long)tot);
+                base = __shfl(base, 0);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                if (keepA) {
+                    const unsigned long long i = base + __popcll(ba & below);
+                    if ((int64_t)i < P.capacity) { surv_w[i] = w; surv_side[i] = 0; }
+                }
+                if (keepB) {
+                    const unsigned long long i = base + __popcll(ba) + __popcll(bb & below);
+                    if ((int64_t)i < P.capacity) { surv_w[i] = w; surv_side[i] = 1; }
+                }
+            }
+            if (live) {
+                ++r;
+                live = r < r1 && successor(w, n, k);
+            }
+        }
+    }
+    atomicAdd(&ctl->words, nwords);
+}
+
+__global__ void k_gen_unpack(const u128* __restrict__ surv_w, int64_t m, int n, int a0, int stride,
+                             int8_t* __restrict__ words) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const u128 w = surv_w[i];
+        int8_t* o = words + i * stride;
+        for (int p = 0; p < stride; ++p) o[p] = p < n ? (int8_t)(a0 + 2 * digit_at(w, n, p)) : 0;
+    }
+}
+
+// FP64 keys of accepted rows: A plain rint(PSD_k), B inverted bound - rint(PSD_k).
+__global__ void k_keys_fp64(const int8_t* __restrict__ words, int stride, int64_t m, int n,
+                            const double* __restrict__ trig, const uint8_t* __restrict__ decision,
+                            const uint8_t* __restrict__ side, int bound, uint8_t* __restrict__ keys,
+                            unsigned int* __restrict__ ties) {
+    const int K = n >> 1;
+    double psd[kMaxK];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        if (decision[i] != kAccept) continue;
+        fp64_psd(words + i * stride, n, trig, trig + n, psd);
+        unsigned int t = 0;
+        for (int k = 0; k < K; ++k) {
+            const double f = psd[k] - floor(psd[k]);
+            if (fabs(f - 0.5) < 1e-9) ++t;
+            const int v = (int)rint(psd[k]);
+            keys[i * kMaxK + k] = (uint8_t)(side[i] ? bound - v : v);
+        }
+        if (t) atomicAdd(ties, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host
+
+namespace {
+
+std::vector<int> units_of(int n) {
+    std::vector<int> u;
+    for (int i = 2; i < n; ++i)
+        if (std::gcd(i, n) == 1) u.push_back(i);
+    return u;
+}
+
+}  // namespace
+
+int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int proc_num, GenOutput& out,
+                   gp_stats* stats) {
+    hipStream_t st = ctx->stream;
+    const int n = order / compress;
+    if (n < 2 || n > 32) throw ArgError("gen: word length " + std::to_string(n) + " outside [2, 32]");
+    const std::vector<int> alpha = getalphabet(compress);
+    const int k = (int)alpha.size();
+    if (k > 16) throw ArgError("gen: alphabet larger than 16 letters");
+    const int a0 = alpha[0];
+    const auto decomps = getdecomps(order * 2 + (order - 1) * paf);
+    if (decomps.empty()) return GP_NO_INPUT;
+    const int bound = order * 2 - paf;
+    if (bound > 255 || bound < 0) throw ArgError("gen: keys need 0 <= 2*order - paf <= 255");
+    // row sum x <-> index sum J = (x - n a0) / 2
+    std::map<int, int> side_of_J;
+    auto addJ = [&](int x, int bit) {
+        int v = x - n * a0;
+        if (v < 0 || (v & 1)) return;
+        int J = v / 2;
+        if (J > n * (k - 1)) return;
+        side_of_J[J] |= bit;
+    };
+    for (auto& d : decomps) { addJ(d.first, 1); addJ(d.second, 2); }
+    // counts cnt[m][j]
+    const int jmax = n * (k - 1);
+    const int cstride = jmax + 1;
+    std::vector<uint64_t> cnt((size_t)(n + 1) * cstride, 0);
+    cnt[0] = 1;
+    const uint64_t cap = (uint64_t)1 << 62;
+    for (int m = 1; m <= n; ++m)
+        for (int j = 0; j <= jmax; ++j) {
+            unsigned __int128 s = 0;
+            for (int d = 0; d < k && d <= j; ++d) s += cnt[(m - 1) * cstride + (j - d)];
+            cnt[m * cstride + j] = s > cap ? cap : (uint64_t)s;
+        }
+    std::vector<GenSeg> segs;
+    std::vector<GenTask> tasks;
+    int64_t total = 0;
+    const int per_lane = 128;
+    for (auto& [J, side] : side_of_J) {
+        GenSeg s{J, side, (int64_t)cnt[n * cstride + J]};
+        if ((uint64_t)s.count >= cap) throw ArgError("gen: search space too large");
+        const int id = (int)segs.size();
+        segs.push_back(s);
+        for (int64_t r = 0; r < s.count; r += 64 * per_lane) tasks.push_back(GenTask{id, per_lane, r, s.count});
+        total += s.count;
+    }
+    GenParams P{};
+    P.n = n;
+    P.k = k;
+    auto ua = units_of(n);
+    if (ua.size() > 64) throw ArgError("gen: too many units");
+    P.nunits_a = (int)ua.size();
+    for (size_t i = 0; i < ua.size(); ++i) P.units_a[i] = ua[i];
+    P.nunits_b = n > 2 ? 1 : 0;
+    P.units_b[0] = n - 1;
+    // job split (lyndon.cpp:87-121): prefix shard table
+    std::vector<uint32_t> bits;
+    int shard_len = 0;
+    if (proc_num > 1) {
+        shard_len = shard_prefix_len(n, k, proc_num);
+        if (shard_len > 0) {
+            uint64_t space = 1;
+            for (int i = 0; i < shard_len; ++i) space *= (uint64_t)k;
+            if (space > ((uint64_t)1 << 31)) throw ArgError("gen: job-split prefix table too large");
+            bits.assign((size_t)(space + 31) / 32, 0);
+            for (uint64_t idx : shard_prefixes(shard_len, alpha, proc_id, proc_num)) bits[idx >> 5] |= 1u << (idx & 31);
+        }
+    }
+    P.shard_len = shard_len;
+    int64_t capacity = std::max<int64_t>(1 << 20, total / 4 + 1024);
+    DevBuf& d_segs = ctx->buf("g_segs");
+    DevBuf& d_tasks = ctx->buf("g_tasks");
+    DevBuf& d_cnt = ctx->buf("g_cnt");
+    DevBuf& d_bits = ctx->buf("g_bits");
+    DevBuf& d_ctl = ctx->buf("g_ctl");
+    GP_HIP(hipMemcpyAsync(d_segs.ensure(segs.size() * sizeof(GenSeg)), segs.data(), segs.size() * sizeof(GenSeg),
+                          hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_tasks.ensure(tasks.size() * sizeof(GenTask) + 16), tasks.data(),
+                          tasks.size() * sizeof(GenTask), hipMemcpyHostToDevice, st));
+    GP_HIP(hipMemcpyAsync(d_cnt.ensure(cnt.size() * 8), cnt.data(), cnt.size() * 8, hipMemcpyHostToDevice, st));
+    if (!bits.empty())
+        GP_HIP(hipMemcpyAsync(d_bits.ensure(bits.size() * 4), bits.data(), bits.size() * 4, hipMemcpyHostToDevice, st));
+    GenCtl h{};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        P.capacity = capacity;
+        u128* d_w = static_cast<u128*>(ctx->buf("g_w").ensure((size_t)capacity * sizeof(u128)));
+        uint8_t* d_side = static_cast<uint8_t*>(ctx->buf("g_side").ensure((size_t)capacity));
+        GP_HIP(hipMemsetAsync(d_ctl.ensure(sizeof(GenCtl)), 0, sizeof(GenCtl), st));
+        GP_HIP(hipEventRecord(ctx->ev0, st));
+        const int64_t waves = std::min<int64_t>((int64_t)tasks.size(), (int64_t)ctx->num_cus * 32);
+        if (!tasks.empty())
+            hipLaunchKernelGGL(k_gen_enum, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0, st, P,
+                               d_segs.as<GenSeg>(), d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(),
+                               cstride, bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side,
+                               d_ctl.as<GenCtl>());
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipEventRecord(ctx->ev1, st));
+        GP_HIP(hipMemcpyAsync(&h, d_ctl.p, sizeof h, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (stats) stats->kernel_ms += ms;
+        if ((int64_t)h.survivors <= capacity) break;
+        capacity = (int64_t)h.survivors + 1024;
+    }
+    const int64_t m = (int64_t)h.survivors;
+    if (stats) { stats->words += h.words; stats->canonical += (uint64_t)m; }
+    // PSD filter on the canonical survivors
+    const int table = ctx->tables.get(n);
+    const PsdTableDesc& d = ctx->tables.desc[table];
+    const int stride = 16 * d.nb;
+    int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)std::max<int64_t>(m, 1) * stride));
+    uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)std::max<int64_t>(m, 1)));
+    std::vector<int8_t> words_h;
+    std::vector<uint8_t> keys_h, dec_h, side_h;
+    if (m > 0) {
+        hipLaunchKernelGGL(k_gen_unpack, dim3(ctx->num_cus * 4), dim3(256), 0, st, ctx->buf("g_w").as<u128>(), m, n, a0,
+                           stride, d_words);
+        GP_HIP(hipGetLastError());
+        std::vector<FilterThr> tt(ctx->tables.desc.size());
+        const double E = ctx->psd_mode == GP_PSD_SPLIT_BF16 ? split_bf16_E(d, (double)n * compress)
+                                                            : fp32_E(n, (double)n * compress);
+        tt[table] = make_thr(2.0 * order - paf + 0.001, E);
+        std::vector<BatchSeg> bs{BatchSeg{table, 0, 0, m}};
+        run_psd_filter(ctx, d_words, stride, m, bs, tt, d_dec, stats);
+        uint8_t* d_keys = static_cast<uint8_t*>(ctx->buf("g_keys").ensure((size_t)m * kMaxK + 16));
+        unsigned int* d_ties = ctx->dev<unsigned int>("g_ties", 1);
+        GP_HIP(hipMemsetAsync(d_ties, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_keys_fp64, dim3(ctx->num_cus * 4), dim3(256), 0, st, d_words, stride, m, n,
+                           ctx->tables.d_trig64.as<double>() + d.trig_off, d_dec, ctx->buf("g_side").as<uint8_t>(), bound,
+                           d_keys, d_ties);
+        GP_HIP(hipGetLastError());
+        words_h.resize((size_t)m * stride);
+        keys_h.resize((size_t)m * kMaxK);
+        dec_h.resize((size_t)m);
+        side_h.resize((size_t)m);
+        unsigned int ties = 0;
+        GP_HIP(hipMemcpyAsync(words_h.data(), d_words, words_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(keys_h.data(), d_keys, keys_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(dec_h.data(), d_dec, dec_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(side_h.data(), ctx->buf("g_side").p, side_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(&ties, d_ties, sizeof ties, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        if (stats) stats->tie_band += ties;
+    }
+    // collect A and B in the reference's file order (lexicographic order of the words)
+    const int K = n / 2;
+    std::vector<int64_t> ia, ib;
+    for (int64_t i = 0; i < m; ++i) {
+        if (dec_h[i] != kAccept) continue;
+        (side_h[i] ? ib : ia).push_back(i);
+    }
+    if (stats) stats->psd_pass += ia.size() + ib.size();
+    auto lexless = [&](int64_t x, int64_t y) {
+        return std::lexicographical_compare(&words_h[x * stride], &words_h[x * stride] + n, &words_h[y * stride],
+                                            &words_h[y * stride] + n);
+    };
+    std::sort(ia.begin(), ia.end(), lexless);
+    std::sort(ib.begin(), ib.end(), lexless);
+    auto emit = [&](const std::vector<int64_t>& idx, CandSet& cs) {
+        cs.n = n;
+        cs.nkey = K;
+        cs.words.resize(idx.size() * n);
+        cs.keys.resize(idx.size() * K);
+        for (size_t r = 0; r < idx.size(); ++r) {
+            std::memcpy(&cs.words[r * n], &words_h[idx[r] * stride], n);
+            std::memcpy(&cs.keys[r * K], &keys_h[idx[r] * kMaxK], K);
+        }
+    };
+    emit(ia, out.a);
+    emit(ib, out.b);
+    return GP_OK;
+}
+
+}  // namespace gp

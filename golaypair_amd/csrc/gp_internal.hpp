@@ -93,6 +93,16 @@ struct gp_ctx {
     gp::DevBuf words, tiles, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     int num_cus = 256;
+    // named scratch buffers of the pipeline stages
+    std::map<std::string, std::unique_ptr<gp::DevBuf>> pool;
+    gp::DevBuf& buf(const std::string& name) {
+        auto& p = pool[name];
+        if (!p) p.reset(new gp::DevBuf());
+        return *p;
+    }
+    template <class T> T* dev(const std::string& name, size_t count) {
+        return static_cast<T*>(buf(name).ensure(count * sizeof(T) + 64));
+    }
 };
 
 namespace gp {

@@ -1,0 +1,32 @@
+// Device radix sort and the candidate join shared by match and uncompress.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+struct gp_ctx;
+struct gp_stats;
+
+namespace gp {
+
+// Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit).
+void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
+                      uint32_t* hist, int64_t n, int begin_bit, int end_bit);
+size_t radix_hist_bytes(int64_t n);
+
+// Join of two candidate sets on equal key tuples (nkey bytes per record,
+// row stride kstride) followed by the exact integer PAF check
+//   PAF_a(s) + PAF_b(s) == paf_target  for s = 1..n/2   (golay.cpp:27-44).
+// Words are int8 rows of stride wstride.  Returns the accepted (ia, ib)
+// index pairs sorted lexicographically.  All pointers are device memory.
+struct JoinResult {
+    std::vector<uint32_t> ia, ib;
+    uint64_t checks = 0;
+};
+JoinResult join_pairs(gp_ctx* ctx, const int8_t* d_wa, const uint8_t* d_ka, int64_t na, const int8_t* d_wb,
+                      const uint8_t* d_kb, int64_t nb, int n, int nkey, int wstride, int kstride, int paf_target,
+                      gp_stats* stats);
+
+}  // namespace gp

@@ -1,0 +1,127 @@
+// LSD radix sort of (uint64 key, uint32 value) pairs, 8 bits per pass.
+//   k_rs_hist     per-block digit histogram in LDS
+//   k_rs_scan     digit-major exclusive scan of all block histograms
+//   k_rs_scatter  stable scatter: each block walks its chunk in 256-element
+//                 tiles; ranks inside a wave come from 8 ballots (one per digit
+//                 bit), wave offsets from LDS counts, block offsets from the scan.
+// HBM traffic per pass: read keys+values twice (histogram, scatter) and write
+// them once: 3 * 12 bytes per element.
+#include <algorithm>
+
+#include "gp_internal.hpp"
+#include "gp_sort.hpp"
+
+namespace gp {
+
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsChunk = kRsThreads * kRsItems;
+
+__global__ void __launch_bounds__(kRsThreads) k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, int nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    for (int i = 0; i < kRsItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kRsThreads + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m entries by one 1024-thread block.
+__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ hist, int64_t m) {
+    __shared__ uint32_t part[1024];
+    const int64_t per = (m + 1023) / 1024;
+    const int64_t b = (int64_t)threadIdx.x * per, e = min(b + per, m);
+    uint32_t s = 0;
+    for (int64_t i = b; i < e; ++i) s += hist[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (int64_t i = b; i < e; ++i) {
+        uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint64_t* __restrict__ keys_in,
+                                                           const uint32_t* __restrict__ vals_in, int64_t n, int shift,
+                                                           const uint32_t* __restrict__ offs, int nblocks,
+                                                           uint64_t* __restrict__ keys_out,
+                                                           uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t run[256];              // running output position per digit
+    __shared__ uint32_t wcnt[kRsThreads / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    run[threadIdx.x] = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int it = 0; it < kRsItems; ++it) {
+        for (int w = 0; w < kRsThreads / 64; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t idx = base + (int64_t)it * kRsThreads + threadIdx.x;
+        const bool valid = idx < n;
+        uint64_t k = 0;
+        uint32_t v = 0;
+        uint32_t dg = 0;
+        if (valid) { k = keys_in[idx]; v = vals_in[idx]; dg = (uint32_t)((k >> shift) & 255u); }
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            const unsigned long long b = __ballot((dg >> bit) & 1u);
+            peers &= ((dg >> bit) & 1u) ? b : ~b;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t off = 0;
+        if (valid) {
+            for (int w = 0; w < wave; ++w) off += wcnt[w][dg];
+            vals_out[run[dg] + off + rank] = v;
+            keys_out[run[dg] + off + rank] = k;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int w = 0; w < kRsThreads / 64; ++w) tot += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+// keys/vals hold the input; the result ends in keys/vals (ping-pong through tmp).
+void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
+                      uint32_t* hist, int64_t n, int begin_bit, int end_bit) {
+    if (n <= 1) return;
+    if (n > 0xFFFFFFFFll) throw ArgError("radix sort: more than 2^32 elements");
+    const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
+    uint64_t* ki = keys;
+    uint32_t* vi = vals;
+    uint64_t* ko = tmp_keys;
+    uint32_t* vo = tmp_vals;
+    int passes = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, st, ki, n, shift, hist, nblocks);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, (int64_t)nblocks * 256);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, st, ki, vi, n, shift, hist, nblocks, ko,
+                           vo);
+        GP_HIP(hipGetLastError());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    if (passes & 1) {
+        GP_HIP(hipMemcpyAsync(keys, ki, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+        GP_HIP(hipMemcpyAsync(vals, vi, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+}
+
+size_t radix_hist_bytes(int64_t n) { return (size_t)((n + kRsChunk - 1) / kRsChunk + 1) * 256 * 4; }
+
+}  // namespace gp

@@ -153,6 +153,15 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
                         const int32_t* orders, const int32_t* pafs, int64_t count,
                         uint8_t* pass, float* max_psd, gp_stats* stats);
 
+/* FP64 key digits of caller words, the numbers printed after "PSD" by
+ * write_seq_psd / write_seq_psd_invert (src/utils/io.cpp:14-36, called at
+ * generate_hybrid.cpp:144,152 and uncompression.cpp:200-204):
+ * keys[i*(n/2) + k-1] = rint(PSD_i(k)), or bound - rint(PSD_i(k)) when
+ * `inverted`, k = 1..n/2.  Row i holds n entries at words + i*stride.  Returns
+ * GP_ERR_ARG if a digit falls outside [0, 255]. */
+int gp_psd_keys(gp_ctx* ctx, const int8_t* words, int stride, int n, int64_t count, int bound, int inverted,
+                uint8_t* keys, gp_stats* stats);
+
 /* Batched PSD microbenchmark on device-generated words: `count` candidates,
  * with mix[j] = {n, c, order, paf, share} rows (share = number of candidates of
  * that kind).  Word i is generated from a splitmix64 counter (seed, i) and its
