@@ -264,7 +264,52 @@ def _ctx_methods():
                                         ptr(keys, ctypes.c_uint8), None), "gp_psd_keys")
         return keys
 
-    for f in (generate, sort_candidates, match, psd_keys):
+    def uncompress_side(self, orig, compress, new_compress, paf, seqflag):
+        """Candidates of one sequence in DFS order; returns (cands, rotated orig, stats)."""
+        o = np.ascontiguousarray(orig, dtype=np.int8).copy()
+        N = len(o) * (compress // new_compress)
+        cap = 1 << 12
+        while True:
+            out = _empty_cands(N, cap)
+            s = out.struct()
+            st = GpStats()
+            o2 = o.copy()
+            rc = self.lib.gp_uncompress_side(self.h, ptr(o2, ctypes.c_int8), len(o2), compress, new_compress, paf,
+                                             seqflag, ctypes.byref(s), ctypes.byref(st))
+            if rc == GP_ERR_CAPACITY:
+                cap = s.count + 1
+                continue
+            self.check(rc, "gp_uncompress_side")
+            return Cands(out.words[:s.count], out.keys[:s.count]), o2, st
+
+    def uncompress(self, order, compress, new_compress, paf, pa, pb, want_tmp=False):
+        """Uncompress pair arrays (m, order/compress); returns (a, b, tmp_a, tmp_b, stats)."""
+        pa = np.ascontiguousarray(pa, dtype=np.int8)
+        pb = np.ascontiguousarray(pb, dtype=np.int8)
+        L = order // compress
+        N = order // new_compress
+        inp = GpPairs(L, len(pa), len(pa), ptr(pa, ctypes.c_int8), ptr(pb, ctypes.c_int8))
+        cap, tcap = 1024, 1 << 12
+        while True:
+            oa = np.zeros((cap, N), np.int8)
+            ob = np.zeros((cap, N), np.int8)
+            out = GpPairs(N, 0, cap, ptr(oa, ctypes.c_int8), ptr(ob, ctypes.c_int8))
+            ta, tb = _empty_cands(N, tcap), _empty_cands(N, tcap)
+            sa, sb = ta.struct(), tb.struct()
+            st = GpStats()
+            rc = self.lib.gp_uncompress(self.h, order, compress, new_compress, paf, ctypes.byref(inp),
+                                        ctypes.byref(out), ctypes.byref(sa) if want_tmp else None,
+                                        ctypes.byref(sb) if want_tmp else None, ctypes.byref(st))
+            if rc == GP_ERR_CAPACITY:
+                cap = max(cap, out.count + 1)
+                tcap = max(tcap, sa.count + 1, sb.count + 1)
+                continue
+            self.check(rc, "gp_uncompress")
+            tmp = (Cands(ta.words[:sa.count], ta.keys[:sa.count]), Cands(tb.words[:sb.count], tb.keys[:sb.count])) \
+                if want_tmp else (None, None)
+            return oa[:out.count], ob[:out.count], tmp[0], tmp[1], st
+
+    for f in (generate, sort_candidates, match, psd_keys, uncompress_side, uncompress):
         setattr(Context, f.__name__, f)
 
 

@@ -127,9 +127,9 @@ int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new
  *   work_dir, const std::string& prefix, DeviceType dev)
  *   (src/core/uncompression/uncompression.h:19-29, uncompression.cpp:32-79).
  * Output pairs are appended in the reference's order (input pair order, then
- * match order).  `tmp` (may be NULL) receives the last pair's sorted A and B
- * candidate sets, the content the reference leaves in its
- * uncompress_tmp_{a,b}.sorted files. */
+ * match order).  tmp_a / tmp_b (may be NULL) receive the last pair's A and B
+ * candidate sets in DFS order, the content the reference leaves in its
+ * uncompress_tmp_{a,b} files (gp_sort_candidates gives the .sorted ones). */
 int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int paf,
                   const gp_pairs* in, gp_pairs* out, gp_cands* tmp_a, gp_cands* tmp_b,
                   gp_stats* stats);

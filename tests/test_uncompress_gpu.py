@@ -1,0 +1,84 @@
+"""GPU uncompression vs the oracle (uncompression.cpp:32-210).
+
+Per sequence: the candidate lines in DFS order (the reference's
+uncompress_tmp_{a,b} content) must be byte-identical; per pair file: the pairs
+at the finer compression must be byte-identical, in order.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import golaypair_amd as g
+from tests import _oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_dir(order, compress, paf, stages):
+    d = tempfile.mkdtemp()
+    subprocess.run([_oracle.CLI, str(order), "-c"] + [str(c) for c in compress] +
+                   ["--paf", str(paf), "-d", d] + stages, check=True, capture_output=True)
+    return os.path.join(d, "order-%d" % order)
+
+
+def _read_pairs(path, L):
+    rows = [list(map(int, l.split())) for l in open(path).read().splitlines() if l.strip()]
+    a = np.array([r[:L] for r in rows], np.int8).reshape(-1, L)
+    b = np.array([r[L:] for r in rows], np.int8).reshape(-1, L)
+    return a, b
+
+
+CASES = [  # order, compress pair, paf: r = 2, 3, 5 and odd/even finer alphabets
+    (16, [2, 1], 0),
+    (26, [2, 1], 0),
+    (21, [3, 1], -2),
+    (27, [3, 1], -2),
+    (20, [5, 1], 0),
+    (16, [4, 2], 0),
+    (20, [10, 5], 0),
+]
+
+
+@pytest.mark.parametrize("order,compress,paf", CASES)
+def test_uncompress_side_matches_oracle(gpu_ctx, order, compress, paf):
+    wd = _oracle_dir(order, compress, paf, ["gen", "sort", "match"])
+    c, c2 = compress
+    L = order // c
+    pa, pb = _read_pairs(os.path.join(wd, "%d-pairs-%d" % (order, c)), L)
+    assert len(pa) > 0
+    for q in range(min(len(pa), 6)):
+        for seqflag, seq in ((0, pa[q]), (1, pb[q])):
+            ref_text, ref_leaves = _oracle.uncompress_side(seq, c, c2, paf, seqflag)
+            cands, rot, st = gpu_ctx.uncompress_side(seq, c, c2, paf, seqflag)
+            got = "".join(l + "\n" for l in cands.lines())
+            assert got == ref_text, (q, seqflag)
+            assert st.psd_decisions == ref_leaves
+
+
+@pytest.mark.parametrize("order,compress,paf", CASES)
+def test_uncompress_pipeline_matches_oracle(gpu_ctx, order, compress, paf):
+    wd = _oracle_dir(order, compress, paf, ["gen", "sort", "match", "uncompress"])
+    c, c2 = compress
+    pa, pb = _read_pairs(os.path.join(wd, "%d-pairs-%d" % (order, c)), order // c)
+    oa, ob, ta, tb, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb, want_tmp=True)
+    got = "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob))
+    assert got == open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
+    # the last pair's temporary candidate files
+    assert "".join(l + "\n" for l in ta.lines()) == open(os.path.join(wd, "uncompress_tmp_a")).read()
+    assert "".join(l + "\n" for l in tb.lines()) == open(os.path.join(wd, "uncompress_tmp_b")).read()
+    assert st.tie_band == 0
+
+
+def test_uncompress_empty_input(gpu_ctx):
+    z = np.zeros((0, 8), np.int8)
+    oa, ob, ta, tb, st = gpu_ctx.uncompress(16, 2, 1, 0, z, z, want_tmp=True)
+    assert len(oa) == 0 and len(ta) == 0
+
+
+def test_uncompress_compress1_is_noop(gpu_ctx):
+    a = np.ones((2, 16), np.int8)
+    oa, ob, _, _, _ = gpu_ctx.uncompress(16, 1, 1, 0, a, a)
+    assert len(oa) == 0
