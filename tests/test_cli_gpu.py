@@ -1,0 +1,83 @@
+"""The drop-in CLI (golaypair_amd/bin/golaypair) vs the oracle CLI: every file of
+the results directory byte-identical (src/main.cpp:14-251 file names and formats),
+same exit codes, and stage-by-stage interop with files the oracle wrote."""
+import filecmp
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+import golaypair_amd as g
+from tests import _oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def run(exe, args, d):
+    # -d before the subcommands: after `uncompress` it means --device (main.cpp:202,220)
+    return subprocess.run([exe, "-d", d] + [str(a) for a in args], capture_output=True, text=True)
+
+
+def same_dirs(d1, d2):
+    f1, f2 = sorted(os.listdir(d1)), sorted(os.listdir(d2))
+    assert f1 == f2
+    for f in f1:
+        assert open(os.path.join(d1, f), "rb").read() == open(os.path.join(d2, f), "rb").read(), f
+
+
+@pytest.mark.parametrize("args", [
+    [16, "--compress", 2, 1, "all"],
+    [26, "--compress", 2, 1, "all"],
+    [21, "--compress", 3, 1, "--paf", -2, "all"],
+    [20, "--compress", 10, 5, 1, "all"],
+    [16, "--compress", 4, 2, 1, "all"],
+    [20, "-c", 5, 1, "all"],
+    [27, "-c", 3, 1, "--paf", -2, "all"],
+])
+def test_cli_all_matches_oracle(args):
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        r1 = run(g.CLI_PATH, args, d1)
+        r2 = run(_oracle.CLI, args, d2)
+        assert r1.returncode == r2.returncode == 0, r1.stderr
+        order = args[0]
+        same_dirs(os.path.join(d1, "order-%d" % order), os.path.join(d2, "order-%d" % order))
+
+
+def test_cli_c1_golden():
+    with tempfile.TemporaryDirectory() as d:
+        assert run(g.CLI_PATH, [16, "--compress", 2, 1, "all"], d).returncode == 0
+        same_dirs(os.path.join(d, "order-16"), os.path.join(GOLDEN, "v16_c2_1"))
+
+
+def test_cli_v34_known_answer():
+    with tempfile.TemporaryDirectory() as d:
+        r = run(g.CLI_PATH, [34, "--compress", 2, 1, "all"], d)
+        assert r.returncode == 0, r.stderr
+        got = open(os.path.join(d, "order-34", "34-unique")).read()
+        assert got.count("\n") == 373
+        assert got == open(os.path.join(GOLDEN, "v34", "34-unique")).read()
+
+
+def test_cli_stage_interop_with_oracle_files():
+    # oracle writes gen shards (-j/-n), our CLI sorts, matches, uncompresses, filters
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        for j in range(3):
+            assert run(_oracle.CLI, [26, "-c", 2, 1, "-j", j, "-n", 3, "gen"], d1).returncode == 0
+            assert run(g.CLI_PATH, [26, "-c", 2, 1, "-j", j, "-n", 3, "gen"], d2).returncode == 0
+        wd1, wd2 = os.path.join(d1, "order-26"), os.path.join(d2, "order-26")
+        same_dirs(wd1, wd2)
+        for stage in ("sort", "match", "uncompress", "filter"):
+            assert run(g.CLI_PATH, [26, "-c", 2, 1, "-n", 3, stage], d1).returncode == 0
+            assert run(_oracle.CLI, [26, "-c", 2, 1, "-n", 3, stage], d2).returncode == 0
+        same_dirs(wd1, wd2)
+
+
+def test_cli_exit_codes():
+    with tempfile.TemporaryDirectory() as d:
+        assert run(g.CLI_PATH, [7, "gen"], d).returncode == 1          # no decomposition
+        assert run(g.CLI_PATH, [16], d).returncode != 0                 # a subcommand is required
+        assert run(g.CLI_PATH, [16, "-c", 2, "match"], d).returncode == 1   # no sorted files
+        r = run(g.CLI_PATH, [16, "-c", 2, 1, "filter"], d)             # missing pairs file
+        assert r.returncode == 0 and "Bad File." in r.stdout
