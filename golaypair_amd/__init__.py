@@ -43,7 +43,14 @@ class GpStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in
                 ("words", "psd_decisions", "psd_pass", "band_rechecks", "key_rechecks", "canonical",
                  "pair_checks", "pairs", "tie_band")] + [("kernel_ms", ctypes.c_double),
-                                                         ("psd_kernel_ms", ctypes.c_double)]
+                                                         ("psd_kernel_ms", ctypes.c_double),
+                                                         ("band_kernel_ms", ctypes.c_double),
+                                                         ("psd_launches", ctypes.c_uint64)]
+
+    def add(self, other):
+        for k, _ in self._fields_:
+            setattr(self, k, getattr(self, k) + getattr(other, k))
+        return self
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -309,7 +316,22 @@ def _ctx_methods():
                 if want_tmp else (None, None)
             return oa[:out.count], ob[:out.count], tmp[0], tmp[1], st
 
-    for f in (generate, sort_candidates, match, psd_keys, uncompress_side, uncompress):
+    def filter(self, order, compress, pa, pb):
+        """Equivalence filter of pair arrays; returns (a, b, stats) in (a, b) order."""
+        pa = np.ascontiguousarray(pa, dtype=np.int8)
+        pb = np.ascontiguousarray(pb, dtype=np.int8)
+        n = order // compress
+        m = len(pa)
+        oa = np.zeros((max(m, 1), n), np.int8)
+        ob = np.zeros((max(m, 1), n), np.int8)
+        inp = GpPairs(n, m, m, ptr(pa, ctypes.c_int8), ptr(pb, ctypes.c_int8))
+        out = GpPairs(n, 0, max(m, 1), ptr(oa, ctypes.c_int8), ptr(ob, ctypes.c_int8))
+        st = GpStats()
+        self.check(self.lib.gp_filter(self.h, order, compress, ctypes.byref(inp), ctypes.byref(out),
+                                      ctypes.byref(st)), "gp_filter")
+        return oa[:out.count], ob[:out.count], st
+
+    for f in (generate, sort_candidates, match, psd_keys, uncompress_side, uncompress, filter):
         setattr(Context, f.__name__, f)
 
 

@@ -86,6 +86,8 @@ int gp_generate(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int 
             stats->tie_band += g_gen_stats.tie_band;
             stats->kernel_ms += g_gen_stats.kernel_ms;
             stats->psd_kernel_ms += g_gen_stats.psd_kernel_ms;
+            stats->band_kernel_ms += g_gen_stats.band_kernel_ms;
+            stats->psd_launches += g_gen_stats.psd_launches;
         }
         int ra = export_cands(g_gen_out.a, out_a);
         int rb = export_cands(g_gen_out.b, out_b);

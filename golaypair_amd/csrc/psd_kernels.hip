@@ -339,7 +339,9 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
         stats->band_rechecks += h.count;
         stats->tie_band += h.ties;
         for (const BatchSeg& sg : segs) stats->psd_decisions += (uint64_t)sg.rows;
-        stats->psd_kernel_ms += ms_main + ms_band;
+        stats->psd_kernel_ms += ms_main;
+        stats->band_kernel_ms += ms_band;
+        stats->psd_launches += 1;
         stats->kernel_ms += ms_main + ms_band;
     }
     return ms_main;

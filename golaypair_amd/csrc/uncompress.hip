@@ -670,7 +670,9 @@ UncResult expand_sides(gp_ctx* ctx, const std::vector<SideHost>& sides, int orde
         stats->psd_pass += (uint64_t)m;
         stats->band_rechecks += h.band;
         stats->tie_band += h.ties;
-        stats->psd_kernel_ms += ms_filter + ms_band;
+        stats->psd_kernel_ms += ms_filter;
+        stats->band_kernel_ms += ms_band;
+        stats->psd_launches += 1;
         stats->kernel_ms += ms_filter + ms_band + ms_keys;
     }
     return res;

@@ -74,7 +74,9 @@ typedef struct gp_stats {
     uint64_t tie_band;       /* FP64 PSD within 1e-9 of a rounding or filter
                                 threshold (0 for a parity-grade run)            */
     double kernel_ms;        /* device time of the call's kernels               */
-    double psd_kernel_ms;    /* device time of its PSD-filter kernels           */
+    double psd_kernel_ms;    /* device time of its PSD-filter kernel launches   */
+    double band_kernel_ms;   /* device time of its FP64 band-recheck launches   */
+    uint64_t psd_launches;   /* PSD-filter kernel launches timed above          */
 } gp_stats;
 
 /* Context.  `device` is the HIP ordinal. */
