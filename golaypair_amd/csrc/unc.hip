@@ -1,0 +1,721 @@
+// Uncompression driver: tables, filter launches, and the batched re-match.
+//
+// Re-match (match_pairs on the two expanded candidate files of each pair,
+// uncompression.cpp:71-76).  For a pair (A, B) that passes check_if_pair at the
+// finer compression c', PSD_A(k) + PSD_B(k) = Q for every k != 0 with
+// Q = sum A^2 + sum B^2 - paf c'; with the keys of the expansion stage (A
+// inverted, B plain, uncompression.cpp:199-204) the key texts are equal exactly
+// when Q = 2v - paf (the argument of match.hip, roles exchanged).  So the pairs
+// the reference emits are the PSD-filter survivors (A, B) of one input pair with
+//   PAF_A(s) + PAF_B(s) = paf c'  (s = 1..N/2)  and  sum A^2 + sum B^2 = 2v - paf + paf c'.
+// Every survivor gets a hash of its exact integer PAF vector (B: of paf c' - PAF
+// and of 2v - paf + paf c' - sum B^2); B rows go into an open-addressing hash table,
+// A rows probe it, and every candidate is verified exactly before it is emitted.
+// FP64 keys are computed for the emitted pairs only (their output order is the
+// text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
+// of each emitted pair is checked (counted in tie_band when it fails).
+#include "unc_kernels.inc"  // opens namespace gp (closed at the end of this file)
+
+// ---------------------------------------------------------------------------
+// FP32 (plain) filter: one leaf per lane, VALU FMAs over the suffix entries.
+
+__global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restrict__ sides, int nsides,
+                                                         int64_t nitems, const int32_t* __restrict__ radix,
+                                                         const float* __restrict__ c32, const float* __restrict__ mid,
+                                                         const int8_t* __restrict__ ent, const float* __restrict__ w32,
+                                                         UncCtl* ctl, uint64_t* __restrict__ surv, int64_t scap,
+                                                         uint64_t* __restrict__ band, int64_t bcap) {
+    __shared__ float xpre[4][64];
+    __shared__ uint64_t wbuf_all[4][kWaveBuf];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t* wbuf = wbuf_all[wv];
+    int wcount = 0;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + wv;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int sid = 0;
+    for (int64_t it = wave; it < nitems; it += nwaves) {
+        while (sid + 1 < nsides && it >= sides[sid + 1].item_base) ++sid;
+        const UncSide& S = sides[sid];
+        const int64_t o = it - S.item_base;
+        const int mt = (S.K + 15) / 16;
+        const int nch = mt * 32;
+        const int nent = (S.L - S.b) * S.r;
+        // X_outer, one channel per lane
+        float xo = 0.f;
+        {
+            int64_t rem = o;
+            for (int j = S.a - 1; j >= 0; --j) {
+                const int R = radix[S.radix_off + j];
+                const int d = (int)(rem % R);
+                rem /= R;
+                if (lane < nch) xo += c32[S.c32_off + ((int64_t)j * S.rmax + d) * nch + lane];
+            }
+        }
+        for (int64_t m = 0; m < S.p_mid; ++m) {
+            if (lane < nch) xpre[wv][lane] = xo + mid[S.mid_off + m * nch + lane];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int64_t leaf0 = (o * S.p_mid + m) * S.p_suf;
+            for (int64_t s0 = 0; s0 < S.p_suf; s0 += 64) {
+                const int64_t s = s0 + lane;
+                const bool valid = s < S.p_suf;
+                float mx = 0.f;
+                if (valid) {
+                    const int8_t* e = ent + S.ent_off + s * nent;
+                    for (int k = 0; k < S.K; ++k) {
+                        const int t = k >> 4, row = k & 15;
+                        float re = xpre[wv][t * 32 + row], im = xpre[wv][t * 32 + 16 + row];
+                        for (int q = 0; q < nent; ++q) {
+                            const float x = (float)e[q];
+                            re = __builtin_fmaf(x, w32[S.w32_off + (int64_t)q * nch + t * 32 + row], re);
+                            im = __builtin_fmaf(x, w32[S.w32_off + (int64_t)q * nch + t * 32 + 16 + row], im);
+                        }
+                        mx = fmaxf(mx, __builtin_fmaf(re, re, im * im));
+                    }
+                }
+                const bool acc_ok = valid && mx < S.thr_lo;
+                const bool in_band = valid && !(mx > S.thr_hi) && !acc_ok;
+                const unsigned long long ba = __ballot(acc_ok);
+                const uint64_t rec = ((uint64_t)sid << 40) | (uint64_t)(leaf0 + s);
+                if (ba) {
+                    if (acc_ok) wbuf[wcount + __popcll(ba & below)] = rec;
+                    wcount += __popcll(ba);
+                    if (wcount > kWaveBuf - 64) flush_wave(wbuf, wcount, lane, ctl, surv, scap);
+                }
+                if (in_band) {
+                    const unsigned long long i = atomicAdd(&ctl->band, 1ull);
+                    if ((int64_t)i < bcap) band[i] = rec;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (wcount) flush_wave(wbuf, wcount, lane, ctl, surv, scap);
+}
+
+// ---------------------------------------------------------------------------
+// survivor rows: PAF-vector hash
+
+__device__ __forceinline__ uint64_t mix_in(uint64_t h, int64_t v) {
+    return (h ^ (uint64_t)(v + 0x100000)) * 0x100000001B3ull + 0x9E3779B97F4A7C15ull;
+}
+
+// Decodes a record into its word (N int8) and, when the side has +1 masks,
+// the word's +1 bit mask.
+__device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf, const int32_t* __restrict__ radix,
+                                                const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
+                                                int8_t* w) {
+    uint64_t mask = 0;
+    for (int j = S.L - 1; j >= 0; --j) {
+        const uint32_t R = (uint32_t)radix[S.radix_off + j];
+        int d;
+        if (leaf < 0x100000000ull) {
+            const uint32_t l32 = (uint32_t)leaf;
+            d = (int)(l32 % R);
+            leaf = l32 / R;
+        } else {
+            d = (int)(leaf % R);
+            leaf /= R;
+        }
+        if (w) {
+            const int8_t* t = tup + S.tup_off + ((int64_t)j * S.rmax + d) * S.r;
+            for (int i = 0; i < S.r; ++i) w[j + S.L * i] = t[i];
+        }
+        if (pm1 && S.pm1_off >= 0) mask |= pm1[S.pm1_off + (int64_t)j * S.rmax + d];
+    }
+    return mask;
+}
+
+__global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
+                           const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
+                           const uint64_t* __restrict__ surv, int64_t m, int paf_target, int q_target,
+                           uint64_t* __restrict__ hashes) {
+    int8_t w[kMaxN];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t rec = surv[i];
+        const UncSide& S = sides[rec >> 40];
+        const int N = S.N;
+        const bool fast = S.pm1_off >= 0;
+        const uint64_t mask = decode_word(S, rec & ((1ull << 40) - 1), radix, tup, pm1, fast ? nullptr : w);
+        const uint64_t full = N >= 64 ? ~0ull : ((1ull << N) - 1);
+        int64_t sq = 0;
+        if (fast) sq = N;
+        else for (int p = 0; p < N; ++p) sq += (int)w[p] * (int)w[p];
+        uint64_t h = 0xC6A4A7935BD1E995ull ^ (uint64_t)S.group;
+        h = mix_in(h, S.role ? (int64_t)q_target - sq : sq);
+        for (int s = 1; s <= N / 2; ++s) {
+            int64_t v;
+            if (fast) {
+                const uint64_t rot = ((mask >> s) | (mask << (N - s))) & full;
+                v = N - 2 * (int)__popcll(mask ^ rot);  // __popcll is unsigned
+            } else {
+                v = 0;
+                int j = s;
+                for (int p = 0; p < N; ++p) {
+                    v += (int)w[p] * (int)w[j];
+                    if (++j == N) j = 0;
+                }
+            }
+            h = mix_in(h, S.role ? (int64_t)paf_target - v : v);
+        }
+        hashes[i] = splitmix64(h);
+    }
+}
+
+constexpr uint64_t kEmpty = ~0ull;
+
+__global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
+                            const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
+                            uint64_t tmask) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        if (sides[surv[i] >> 40].role != 1) continue;
+        const uint64_t h = hashes[i];
+        const unsigned long long e = ((h & 0xFFFFFFFFull) << 32) | (uint64_t)(uint32_t)i;
+        uint64_t slot = (h >> 24) & tmask;
+        while (atomicCAS(&table[slot], kEmpty, e) != kEmpty) slot = (slot + 1) & tmask;
+    }
+}
+
+struct JoinCtl2 {
+    unsigned long long out;
+    unsigned long long checks;
+};
+
+__global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
+                           const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                           const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
+                           uint64_t tmask, int paf_target, int q_target, uint32_t* __restrict__ out_a,
+                           uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl) {
+    int8_t wa[kMaxN], wb[kMaxN];
+    unsigned long long checks = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t ra = surv[i];
+        const UncSide& SA = sides[ra >> 40];
+        if (SA.role != 0) continue;
+        const uint64_t h = hashes[i];
+        uint64_t slot = (h >> 24) & tmask;
+        bool built = false;
+        for (;;) {
+            const unsigned long long e = table[slot];
+            if (e == kEmpty) break;
+            if ((e >> 32) == (h & 0xFFFFFFFFull)) {
+                const uint32_t j = (uint32_t)(e & 0xFFFFFFFFull);
+                const uint64_t rb = surv[j];
+                const UncSide& SB = sides[rb >> 40];
+                if (hashes[j] == h && SB.group == SA.group) {
+                    if (!built) { decode_word(SA, ra & ((1ull << 40) - 1), radix, tup, nullptr, wa); built = true; }
+                    decode_word(SB, rb & ((1ull << 40) - 1), radix, tup, nullptr, wb);
+                    ++checks;
+                    const int N = SA.N;
+                    int sq = 0;
+                    for (int p = 0; p < N; ++p) sq += (int)wa[p] * wa[p] + (int)wb[p] * wb[p];
+                    bool ok = sq == q_target;
+                    for (int s = 1; ok && s <= N / 2; ++s) {
+                        int acc = 0, k = s;
+                        for (int p = 0; p < N; ++p) {
+                            acc += (int)wa[p] * wa[k] + (int)wb[p] * wb[k];
+                            if (++k == N) k = 0;
+                        }
+                        ok = acc == paf_target;
+                    }
+                    if (ok) {
+                        const unsigned long long o = atomicAdd(&ctl->out, 1ull);
+                        if ((int64_t)o < cap) { out_a[o] = (uint32_t)i; out_b[o] = j; }
+                    }
+                }
+            }
+            slot = (slot + 1) & tmask;
+        }
+    }
+    atomicAdd(&ctl->checks, checks);
+}
+
+// Emitted rows -> words + FP64 keys (A inverted, B plain).
+__global__ void k_emit_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
+                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                            const uint32_t* __restrict__ rows, int64_t m, const double* __restrict__ trig, int bound,
+                            int8_t* __restrict__ words, uint8_t* __restrict__ keys, int32_t* __restrict__ groups,
+                            unsigned int* __restrict__ ties) {
+    double psd[kMaxK];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t rec = surv[rows[i]];
+        const UncSide& S = sides[rec >> 40];
+        groups[i] = S.group;
+        int8_t* w = words + i * S.N;
+        decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, w);
+        fp64_psd(w, S.N, trig, trig + S.N, psd);
+        unsigned int t = 0;
+        for (int k = 0; k < S.K; ++k) {
+            const double f = psd[k] - floor(psd[k]);
+            if (fabs(f - 0.5) < 1e-9) ++t;
+            const int v = (int)rint(psd[k]);
+            keys[i * S.K + k] = (uint8_t)(S.seqflag ? v : bound - v);
+        }
+        if (t) atomicAdd(ties, t);
+    }
+}
+
+// Records of one table side (unordered) -> compact list.
+__global__ void k_select_side(const uint64_t* __restrict__ surv, int64_t m, uint64_t sid,
+                              unsigned long long* __restrict__ count, uint64_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = surv[i];
+        if ((r >> 40) == sid) out[atomicAdd(count, 1ull)] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: tables
+
+namespace {
+
+uint16_t bf16_rne_u(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+double bf16_d(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct Tables {
+    std::vector<UncSide> sides;
+    std::vector<int32_t> radix;
+    std::vector<int8_t> tup, ent;
+    std::vector<float> c32, mid, w32;
+    std::vector<uint4> sufb, frags;
+    std::vector<double> c64;
+    std::vector<uint64_t> pm1;
+    int64_t items = 0;
+};
+
+void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const std::vector<double>& cs,
+              const std::vector<double>& sn, int group, bool fp32) {
+    UncSide D{};
+    D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
+    D.group = group;
+    D.role = S.seqflag ? 1 : 0;
+    const int L = S.L, r = S.r, N = S.N, K = D.K;
+    const int mt = (K + 15) / 16;
+    std::vector<int64_t> R(L);
+    for (int j = 0; j < L; ++j) R[j] = (int64_t)S.T[j].size();
+    // split: suffix [b, L) with p_suf >= 64, mid [a, b) with 16 <= p_mid <= 4096
+    int b = L;
+    int64_t ps = 1;
+    while (b > 0 && ps < 64) { --b; ps *= R[b]; }
+    int a = b;
+    int64_t pm = 1;
+    while (a > 0 && pm < 16 && pm * R[a - 1] <= 4096) { --a; pm *= R[a]; }
+    int64_t po = 1;
+    for (int j = 0; j < a; ++j) po *= R[j];
+    D.a = a; D.b = b; D.p_suf = ps; D.p_mid = pm; D.p_out = po; D.leaves = (int64_t)S.leaves;
+    const int nent = (L - b) * r;
+    D.nbs = std::max(1, (nent + 15) / 16);
+    D.ntiles = (int)((ps + 31) / 32);
+    D.item_base = T.items;
+    T.items += po;
+    D.radix_off = (int64_t)T.radix.size();
+    for (int j = 0; j < L; ++j) T.radix.push_back((int32_t)R[j]);
+    D.tup_off = (int64_t)T.tup.size();
+    T.tup.resize(T.tup.size() + (size_t)L * rmax * r, 0);
+    bool pm1_ok = N <= 64;
+    for (int j = 0; j < L; ++j)
+        for (int64_t d = 0; d < R[j]; ++d)
+            for (int i = 0; i < r; ++i) {
+                const int v = S.T[j][d][i];
+                T.tup[D.tup_off + ((int64_t)j * rmax + d) * r + i] = (int8_t)v;
+                if (v != 1 && v != -1) pm1_ok = false;
+            }
+    D.pm1_off = -1;
+    if (pm1_ok) {
+        D.pm1_off = (int64_t)T.pm1.size();
+        T.pm1.resize(T.pm1.size() + (size_t)L * rmax, 0);
+        for (int j = 0; j < L; ++j)
+            for (int64_t d = 0; d < R[j]; ++d) {
+                uint64_t msk = 0;
+                for (int i = 0; i < r; ++i)
+                    if (S.T[j][d][i] == 1) msk |= 1ull << (j + L * i);
+                T.pm1[D.pm1_off + (int64_t)j * rmax + d] = msk;
+            }
+    }
+    // FP64 per-position contributions and FP32 channel copies
+    D.c64_off = (int64_t)T.c64.size();
+    T.c64.resize(T.c64.size() + (size_t)L * rmax * 2 * K, 0.0);
+    D.c32_off = (int64_t)T.c32.size();
+    T.c32.resize(T.c32.size() + (size_t)L * rmax * mt * 32, 0.f);
+    for (int j = 0; j < L; ++j)
+        for (int64_t d = 0; d < R[j]; ++d)
+            for (int k = 1; k <= K; ++k) {
+                double re = 0, im = 0;
+                for (int i = 0; i < r; ++i) {
+                    const int p = j + L * i;
+                    const int mm = (int)(((int64_t)k * p) % N);
+                    re += S.T[j][d][i] * cs[mm];
+                    im += S.T[j][d][i] * sn[mm];
+                }
+                double* c = &T.c64[D.c64_off + (((int64_t)j * rmax + d) * K + (k - 1)) * 2];
+                c[0] = re; c[1] = im;
+                const int t = (k - 1) / 16, row = (k - 1) % 16;
+                T.c32[D.c32_off + ((int64_t)j * rmax + d) * mt * 32 + t * 32 + row] = (float)re;
+                T.c32[D.c32_off + ((int64_t)j * rmax + d) * mt * 32 + t * 32 + 16 + row] = (float)im;
+            }
+    // mid table: FP64 sums over the middle digits, rounded once to FP32
+    D.mid_off = (int64_t)T.mid.size();
+    T.mid.resize(T.mid.size() + (size_t)pm * mt * 32, 0.f);
+    {
+        std::vector<double> acc((size_t)2 * K);
+        std::vector<int> dg(L, 0);
+        for (int64_t m = 0; m < pm; ++m) {
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (int j = a; j < b; ++j) {
+                const double* c = &T.c64[D.c64_off + ((int64_t)j * rmax + dg[j]) * K * 2];
+                for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
+            }
+            for (int k = 1; k <= K; ++k) {
+                const int t = (k - 1) / 16, row = (k - 1) % 16;
+                T.mid[D.mid_off + m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
+                T.mid[D.mid_off + m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
+            }
+            for (int j = b - 1; j >= a; --j) {  // next mid combination (last position fastest)
+                if (++dg[j] < R[j]) break;
+                dg[j] = 0;
+            }
+        }
+    }
+    // suffix combinations: BF16 B operands (split path) or int8 entries (FP32 path)
+    std::vector<int> sdg(L, 0);
+    std::vector<std::vector<int>> sufvals((size_t)ps, std::vector<int>(nent, 0));
+    for (int64_t s = 0; s < ps; ++s) {
+        for (int e = 0; e < nent; ++e) sufvals[s][e] = S.T[b + e / r][sdg[b + e / r]][e % r];
+        for (int j = L - 1; j >= b; --j) {
+            if (++sdg[j] < R[j]) break;
+            sdg[j] = 0;
+        }
+    }
+    D.sufb_off = (int64_t)T.sufb.size();
+    D.ent_off = (int64_t)T.ent.size();
+    D.w32_off = (int64_t)T.w32.size();
+    if (!fp32) {
+        for (int tile = 0; tile < D.ntiles; ++tile)
+            for (int blk = 0; blk < D.nbs; ++blk)
+                for (int lane = 0; lane < 64; ++lane) {
+                    uint16_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    const int64_t s = (int64_t)tile * 32 + lane % 32;
+                    if (s < ps)
+                        for (int i8 = 0; i8 < 8; ++i8) {
+                            const int e = 16 * blk + 8 * (lane / 32) + i8;
+                            if (e < nent) hv[i8] = bf16_rne_u((float)sufvals[s][e]);
+                        }
+                    uint4 q;
+                    q.x = hv[0] | ((uint32_t)hv[1] << 16);
+                    q.y = hv[2] | ((uint32_t)hv[3] << 16);
+                    q.z = hv[4] | ((uint32_t)hv[5] << 16);
+                    q.w = hv[6] | ((uint32_t)hv[7] << 16);
+                    T.sufb.push_back(q);
+                }
+    } else {
+        for (int64_t s = 0; s < ps; ++s)
+            for (int e = 0; e < nent; ++e) T.ent.push_back((int8_t)sufvals[s][e]);
+        for (int e = 0; e < nent; ++e) {
+            const int j = b + e / r, i = e % r, p = j + L * i;
+            for (int t = 0; t < mt; ++t)
+                for (int row = 0; row < 32; ++row) {
+                    const int k = 16 * t + row % 16 + 1;
+                    float v = 0.f;
+                    if (k <= K) {
+                        const int mm = (int)(((int64_t)k * p) % N);
+                        v = (float)(row >= 16 ? sn[mm] : cs[mm]);
+                    }
+                    T.w32.push_back(v);
+                }
+        }
+    }
+    // A fragments: suffix DFT weights, BF16 hi and lo
+    D.frag_off = (int64_t)T.frags.size();
+    double rho = 0;
+    if (!fp32)
+        for (int t = 0; t < mt; ++t)
+            for (int st = 0; st < 2 * D.nbs; ++st)
+                for (int lane = 0; lane < 64; ++lane) {
+                    uint16_t hv[8];
+                    for (int i8 = 0; i8 < 8; ++i8) {
+                        const int row = lane % 32;
+                        const int e = 16 * (st % D.nbs) + 8 * (lane / 32) + i8;
+                        const int part = st / D.nbs;
+                        const int k = 16 * t + row % 16 + 1;
+                        uint16_t v = 0;
+                        if (e < nent && k <= K) {
+                            const int j = b + e / r, i = e % r;
+                            const int p = j + L * i;
+                            const int mm = (int)(((int64_t)k * p) % N);
+                            const double w = row >= 16 ? sn[mm] : cs[mm];
+                            const uint16_t hi = bf16_rne_u((float)w);
+                            const double r1 = w - bf16_d(hi);
+                            const uint16_t lo = bf16_rne_u((float)r1);
+                            rho = std::max(rho, std::fabs(r1 - bf16_d(lo)));
+                            v = part ? lo : hi;
+                        }
+                        hv[i8] = v;
+                    }
+                    uint4 q;
+                    q.x = hv[0] | ((uint32_t)hv[1] << 16);
+                    q.y = hv[2] | ((uint32_t)hv[3] << 16);
+                    q.z = hv[4] | ((uint32_t)hv[5] << 16);
+                    q.w = hv[6] | ((uint32_t)hv[7] << 16);
+                    T.frags.push_back(q);
+                }
+    // error bound of the filter's Re/Im
+    double s1_pre = 0, s1_suf = 0;
+    for (int j = 0; j < L; ++j) {
+        double mxabs = 0;
+        for (int64_t d = 0; d < R[j]; ++d) {
+            double ab = 0;
+            for (int i = 0; i < r; ++i) ab += std::abs(S.T[j][d][i]);
+            mxabs = std::max(mxabs, ab);
+        }
+        (j < b ? s1_pre : s1_suf) += mxabs;
+    }
+    const double u24 = 0x1p-24, u23 = 0x1p-23;
+    double E;
+    if (!fp32)
+        E = (a + 3) * u24 * s1_pre * 1.01                                   // X_outer + Mid rounding (FP32)
+            + s1_suf * rho * 1.0000001                                      // BF16 split residual
+            + 1.004 * (32.0 * D.nbs + 1.0) * u23 * (s1_pre + s1_suf);       // MFMA accumulation
+    else
+        E = (a + 3) * u24 * s1_pre * 1.01 + s1_suf * u24 * 1.01              // FP32 weights
+            + (nent + 1.0) * u24 * (s1_pre + s1_suf) * 1.01;                // FP32 FMAs
+    const double thr = 2.0 * order - paf + 0.001;
+    const double eps = psd_eps(E, thr);
+    D.thr = thr;
+    D.thr_hi = std::nextafter((float)(thr + eps), INFINITY);
+    D.thr_lo = std::nextafter((float)(thr - eps), -INFINITY);
+    T.sides.push_back(D);
+}
+
+template <class V>
+V* upload(gp_ctx* ctx, const char* name, const std::vector<V>& v) {
+    void* p = ctx->buf(name).ensure(v.size() * sizeof(V) + 64);
+    if (!v.empty()) GP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, ctx->stream));
+    return static_cast<V*>(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host: the three stages
+
+UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::vector<int>& group, int order,
+                    int paf, gp_stats* stats) {
+    UncBatch B;
+    if (sides.empty()) return B;
+    hipStream_t st = ctx->stream;
+    const int N = sides[0].N;
+    if (N > kMaxN || N < 2) throw ArgError("uncompress: expanded length outside [2, 64]");
+    if ((int64_t)sides.size() >= (1ll << 23)) throw ArgError("uncompress: too many sequences in one batch");
+    const int bound = 2 * order - paf;
+    if (bound < 0 || bound > 255) throw ArgError("uncompress: keys need 0 <= 2*order - paf <= 255");
+    const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
+    int rmax = 1;
+    for (const auto& s : sides)
+        for (const auto& t : s.T) rmax = std::max<int>(rmax, (int)t.size());
+    const int table = ctx->tables.get(N);
+    ctx->tables.upload(st);
+    const PsdTableDesc& td = ctx->tables.desc[table];
+    std::vector<double> cs(ctx->tables.trig64.begin() + td.trig_off, ctx->tables.trig64.begin() + td.trig_off + N);
+    std::vector<double> sn(ctx->tables.trig64.begin() + td.trig_off + N,
+                           ctx->tables.trig64.begin() + td.trig_off + 2 * N);
+    Tables T;
+    uint64_t total_leaves = 0;
+    for (size_t i = 0; i < sides.size(); ++i) {
+        if (sides[i].leaves == 0) continue;
+        add_side(T, sides[i], rmax, order, paf, cs, sn, group[i], fp32);
+        B.side_of.push_back((int)i);
+        total_leaves += sides[i].leaves;
+    }
+    B.N = N;
+    B.K = N / 2;
+    B.trig_off = td.trig_off;
+    B.bound = bound;
+    if (T.sides.empty()) return B;
+    B.d_sides = upload(ctx, "u_sides", T.sides);
+    B.d_radix = upload(ctx, "u_radix", T.radix);
+    B.d_tup = upload(ctx, "u_tup", T.tup);
+    B.d_pm1 = upload(ctx, "u_pm1", T.pm1);
+    B.d_c64 = upload(ctx, "u_c64", T.c64);
+    auto* d_c32 = upload(ctx, "u_c32", T.c32);
+    auto* d_mid = upload(ctx, "u_mid", T.mid);
+    auto* d_sufb = upload(ctx, "u_sufb", T.sufb);
+    auto* d_frags = upload(ctx, "u_frags", T.frags);
+    auto* d_ent = upload(ctx, "u_ent", T.ent);
+    auto* d_w32 = upload(ctx, "u_w32", T.w32);
+    std::vector<int64_t> toff{td.trig_off};
+    auto* d_toff = upload(ctx, "u_toff", toff);
+    B.nsides_table = (int)T.sides.size();
+    UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
+    int64_t scap = (int64_t)std::min<uint64_t>(total_leaves * 3 / 5 + (1 << 20), (uint64_t)3 << 30);
+    int64_t bcap = (int64_t)std::min<uint64_t>(total_leaves / 64 + (1 << 16), (uint64_t)1 << 28);
+    UncCtl h{};
+    float ms_filter = 0, ms_band = 0;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        uint64_t* d_surv = ctx->dev<uint64_t>("u_surv", (size_t)scap);
+        uint64_t* d_band = ctx->dev<uint64_t>("u_band", (size_t)bcap);
+        GP_HIP(hipMemsetAsync(ctl, 0, sizeof(UncCtl), st));
+        GP_HIP(hipEventRecord(ctx->ev0, st));
+        const int64_t waves = std::min<int64_t>(T.items, (int64_t)ctx->num_cus * 8);
+        const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+        if (!fp32)
+            hipLaunchKernelGGL(k_unc_filter, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items, B.d_radix,
+                               d_c32, d_mid, d_sufb, d_frags, ctl, d_surv, scap, d_band, bcap);
+        else
+            hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items,
+                               B.d_radix, d_c32, d_mid, d_ent, d_w32, ctl, d_surv, scap, d_band, bcap);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipEventRecord(ctx->ev1, st));
+        GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        GP_HIP(hipEventElapsedTime(&ms_filter, ctx->ev0, ctx->ev1));
+        if ((int64_t)h.band > bcap) { bcap = (int64_t)h.band + 1024; continue; }
+        if (h.band) {
+            hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup,
+                               ctx->tables.d_trig64.as<double>(), d_toff, d_band, ctl, d_surv, scap, (int64_t)h.band);
+            GP_HIP(hipGetLastError());
+            GP_HIP(hipEventRecord(ctx->ev2, st));
+            GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
+            GP_HIP(hipStreamSynchronize(st));
+            GP_HIP(hipEventElapsedTime(&ms_band, ctx->ev1, ctx->ev2));
+        }
+        if ((int64_t)h.surv > scap) { scap = (int64_t)h.surv + 1024; continue; }
+        break;
+    }
+    B.nsurv = (int64_t)h.surv;
+    B.d_surv = ctx->buf("u_surv").as<uint64_t>();
+    if (stats) {
+        stats->words += total_leaves;
+        stats->psd_decisions += total_leaves;
+        stats->psd_pass += (uint64_t)B.nsurv;
+        stats->band_rechecks += h.band;
+        stats->tie_band += h.ties;
+        stats->psd_kernel_ms += ms_filter;
+        stats->band_kernel_ms += ms_band;
+        stats->psd_launches += 1;
+        stats->kernel_ms += ms_filter + ms_band;
+    }
+    return B;
+}
+
+UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* stats) {
+    UncEmitted E;
+    E.N = B.N;
+    E.K = B.K;
+    const int64_t m = B.nsurv;
+    if (m == 0) return E;
+    if (m > 0xFFFFFFFFll) throw ArgError("uncompress: more than 2^32 survivors in one batch");
+    hipStream_t st = ctx->stream;
+    const int q_target = B.bound + paf_target;  // sum A^2 + sum B^2 of an emitted pair
+    uint64_t* hashes = ctx->dev<uint64_t>("u_hash", (size_t)m);
+    uint64_t T = 1;
+    while (T < (uint64_t)m * 2) T <<= 1;  // at most half full even if every row is a B row
+    unsigned long long* table = ctx->dev<unsigned long long>("u_ht", (size_t)T);
+    JoinCtl2* ctl = ctx->dev<JoinCtl2>("u_jctl", 1);
+    GP_HIP(hipEventRecord(ctx->ev0, st));
+    GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
+    const unsigned grid = (unsigned)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
+                       paf_target, q_target, hashes);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1);
+    GP_HIP(hipGetLastError());
+    int64_t cap = 1 << 16;
+    JoinCtl2 h{};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        uint32_t* oa = ctx->dev<uint32_t>("u_oa", (size_t)cap);
+        uint32_t* ob = ctx->dev<uint32_t>("u_ob", (size_t)cap);
+        GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
+        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_surv, hashes,
+                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        if ((int64_t)h.out <= cap) break;
+        cap = (int64_t)h.out;
+    }
+    const int64_t ne = (int64_t)h.out;
+    E.count = ne;
+    if (ne > 0) {
+        // rows of the emitted pairs: A rows then B rows
+        uint32_t* rows = ctx->dev<uint32_t>("u_erows", (size_t)2 * ne);
+        GP_HIP(hipMemcpyAsync(rows, ctx->buf("u_oa").p, ne * 4, hipMemcpyDeviceToDevice, st));
+        GP_HIP(hipMemcpyAsync(rows + ne, ctx->buf("u_ob").p, ne * 4, hipMemcpyDeviceToDevice, st));
+        int8_t* w = ctx->dev<int8_t>("u_ew", (size_t)2 * ne * B.N);
+        uint8_t* k = ctx->dev<uint8_t>("u_ek", (size_t)2 * ne * B.K);
+        int32_t* grp = ctx->dev<int32_t>("u_egrp", (size_t)2 * ne);
+        unsigned int* ties = ctx->dev<unsigned int>("u_eties", 1);
+        GP_HIP(hipMemsetAsync(ties, 0, 4, st));
+        hipLaunchKernelGGL(k_emit_rows, dim3((unsigned)std::min<int64_t>((2 * ne + 127) / 128, 4096)), dim3(128), 0, st,
+                           B.d_sides, B.d_radix, B.d_tup, B.d_surv, rows, 2 * ne,
+                           ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, w, k, grp, ties);
+        GP_HIP(hipGetLastError());
+        E.words.resize((size_t)2 * ne * B.N);
+        E.keys.resize((size_t)2 * ne * B.K);
+        std::vector<int32_t> g2((size_t)2 * ne);
+        unsigned int ht = 0;
+        GP_HIP(hipMemcpyAsync(E.words.data(), w, E.words.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(E.keys.data(), k, E.keys.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(g2.data(), grp, g2.size() * 4, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(&ht, ties, 4, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        E.group.assign(g2.begin(), g2.begin() + ne);
+        if (stats) stats->tie_band += ht;
+    }
+    GP_HIP(hipEventRecord(ctx->ev1, st));
+    GP_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) { stats->pair_checks += h.checks; stats->kernel_ms += ms; }
+    return E;
+}
+
+CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side) {
+    CandSet c;
+    c.n = B.N;
+    c.nkey = B.K;
+    int tsid = -1;
+    for (size_t t = 0; t < B.side_of.size(); ++t)
+        if (B.side_of[t] == input_side) tsid = (int)t;
+    if (tsid < 0 || B.nsurv == 0) return c;
+    hipStream_t st = ctx->stream;
+    unsigned long long* cnt = ctx->dev<unsigned long long>("u_scnt", 1);
+    uint64_t* sel = ctx->dev<uint64_t>("u_sel", (size_t)B.nsurv);
+    GP_HIP(hipMemsetAsync(cnt, 0, 8, st));
+    hipLaunchKernelGGL(k_select_side, dim3(ctx->num_cus * 4), dim3(256), 0, st, B.d_surv, B.nsurv, (uint64_t)tsid, cnt,
+                       sel);
+    GP_HIP(hipGetLastError());
+    unsigned long long m = 0;
+    GP_HIP(hipMemcpyAsync(&m, cnt, 8, hipMemcpyDeviceToHost, st));
+    GP_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> recs(m);
+    if (m) GP_HIP(hipMemcpyAsync(recs.data(), sel, m * 8, hipMemcpyDeviceToHost, st));
+    GP_HIP(hipStreamSynchronize(st));
+    std::sort(recs.begin(), recs.end());  // DFS order
+    if (m == 0) return c;
+    GP_HIP(hipMemcpyAsync(sel, recs.data(), m * 8, hipMemcpyHostToDevice, st));
+    int8_t* w = ctx->dev<int8_t>("u_cw", (size_t)m * B.N);
+    uint8_t* k = ctx->dev<uint8_t>("u_ck", (size_t)m * B.K);
+    UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
+    hipLaunchKernelGGL(k_unc_keys, dim3(ctx->num_cus * 4), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_c64,
+                       sel, (int64_t)m, B.bound, w, k, ctl);
+    GP_HIP(hipGetLastError());
+    c.words.resize((size_t)m * B.N);
+    c.keys.resize((size_t)m * B.K);
+    GP_HIP(hipMemcpyAsync(c.words.data(), w, c.words.size(), hipMemcpyDeviceToHost, st));
+    GP_HIP(hipMemcpyAsync(c.keys.data(), k, c.keys.size(), hipMemcpyDeviceToHost, st));
+    GP_HIP(hipStreamSynchronize(st));
+    return c;
+}
+
+}  // namespace gp

@@ -58,12 +58,17 @@ def test_uncompress_side_matches_oracle(gpu_ctx, order, compress, paf):
             assert st.psd_decisions == ref_leaves
 
 
+@pytest.mark.parametrize("mode", [g.GP_PSD_SPLIT_BF16, g.GP_PSD_FP32])
 @pytest.mark.parametrize("order,compress,paf", CASES)
-def test_uncompress_pipeline_matches_oracle(gpu_ctx, order, compress, paf):
+def test_uncompress_pipeline_matches_oracle(gpu_ctx, order, compress, paf, mode):
     wd = _oracle_dir(order, compress, paf, ["gen", "sort", "match", "uncompress"])
     c, c2 = compress
     pa, pb = _read_pairs(os.path.join(wd, "%d-pairs-%d" % (order, c)), order // c)
-    oa, ob, ta, tb, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb, want_tmp=True)
+    gpu_ctx.set_mode(mode)
+    try:
+        oa, ob, ta, tb, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb, want_tmp=True)
+    finally:
+        gpu_ctx.set_mode(g.GP_PSD_SPLIT_BF16)
     got = "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob))
     assert got == open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
     # the last pair's temporary candidate files
