@@ -296,7 +296,7 @@ def _ctx_methods():
         L = order // compress
         N = order // new_compress
         inp = GpPairs(L, len(pa), len(pa), ptr(pa, ctypes.c_int8), ptr(pb, ctypes.c_int8))
-        cap, tcap = 1024, 1 << 12
+        cap, tcap = max(1024, 4 * len(pa)), 1 << 16
         while True:
             oa = np.zeros((cap, N), np.int8)
             ob = np.zeros((cap, N), np.int8)

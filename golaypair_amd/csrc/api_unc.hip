@@ -80,6 +80,21 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         PairSet inp = import_pairs(in);
         const int64_t P = inp.size();
         const int N = order / new_compress;
+        // a call repeated after GP_ERR_CAPACITY returns the stored result
+        uint64_t key = 1469598103934665603ull;
+        for (int v : {order, compress, new_compress, paf, ctx->psd_mode, (int)P, tmp_a ? 1 : 0, tmp_b ? 1 : 0})
+            key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
+        for (int8_t v : inp.a) key = (key ^ (uint8_t)v) * 1099511628211ull;
+        for (int8_t v : inp.b) key = (key ^ (uint8_t)v) * 1099511628211ull;
+        if (ctx->unc_cache_valid && ctx->unc_cache_key == key) {
+            ctx->unc_cache_valid = false;
+            int rc = GP_OK;
+            if (P > 0 && tmp_a) rc = export_cands(ctx->unc_cache_ta, tmp_a);
+            if (rc == GP_OK && P > 0 && tmp_b) rc = export_cands(ctx->unc_cache_tb, tmp_b);
+            if (rc == GP_OK) rc = export_pairs(ctx->unc_cache_out, out, true);
+            if (rc == GP_ERR_CAPACITY) ctx->unc_cache_valid = true;
+            return rc;
+        }
         PairSet result;
         result.n = N;
         CandSet last_a, last_b;
@@ -87,7 +102,10 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         last_a.nkey = last_b.nkey = N / 2;
         // build all sides up front (host threads), then batch by leaf budget
         std::vector<SideHost> all;
-        build_sides(inp, L, compress, new_compress, 0, P, all);
+        {
+            Phase ph("uncompress: build sides");
+            build_sides(inp, L, compress, new_compress, 0, P, all);
+        }
         int64_t q0 = 0;
         while (q0 < P) {
             uint64_t leaves = 0;
@@ -101,8 +119,17 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             std::vector<SideHost> sides(all.begin() + 2 * q0, all.begin() + 2 * q1);
             std::vector<int> group(sides.size());
             for (size_t i = 0; i < sides.size(); ++i) group[i] = (int)(i / 2);
-            UncBatch B = unc_filter(ctx, sides, group, order, paf, stats);
-            UncEmitted E = unc_join(ctx, B, paf * new_compress, stats);
+            UncBatch B;
+            {
+                Phase ph("uncompress: filter batch");
+                B = unc_filter(ctx, sides, group, order, paf, stats);
+            }
+            UncEmitted E;
+            {
+                Phase ph("uncompress: join batch");
+                E = unc_join(ctx, B, paf * new_compress, stats);
+            }
+            Phase ph_order("uncompress: order + tmp");
             if (q1 == P && (tmp_a || tmp_b)) {  // the last pair's candidate files
                 last_a = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0)));
                 last_b = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0) + 1));
@@ -134,14 +161,28 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             if (stats) stats->pairs += ord.size();
             q0 = q1;
         }
+        int rc = GP_OK;
         if (P > 0) {
-            if (tmp_a) { int rc = export_cands(last_a, tmp_a); if (rc) return rc; }
-            if (tmp_b) { int rc = export_cands(last_b, tmp_b); if (rc) return rc; }
+            if (tmp_a) rc = export_cands(last_a, tmp_a);
+            if (rc == GP_OK && tmp_b) rc = export_cands(last_b, tmp_b);
         } else {
             if (tmp_a) tmp_a->count = 0;
             if (tmp_b) tmp_b->count = 0;
         }
-        return export_pairs(result, out, true);
+        const int64_t base = out->count;
+        if (rc == GP_OK) rc = export_pairs(result, out, true);
+        if (rc == GP_ERR_CAPACITY) {
+            if (out->count < base + result.size()) out->count = base + result.size();
+            // report every size the caller must grow, then keep the result for the retry
+            if (tmp_a && P > 0) tmp_a->count = last_a.size();
+            if (tmp_b && P > 0) tmp_b->count = last_b.size();
+            ctx->unc_cache_key = key;
+            ctx->unc_cache_out = std::move(result);
+            ctx->unc_cache_ta = std::move(last_a);
+            ctx->unc_cache_tb = std::move(last_b);
+            ctx->unc_cache_valid = true;
+        }
+        return rc;
     });
 }
 

@@ -408,7 +408,7 @@ int main(int argc, char** argv) {
             Cands ta, tb;
             ta.n = tb.n = N;
             gp_stats st{};
-            int64_t cap = 1024, tcap = 1 << 12;
+            int64_t cap = std::max<int64_t>(1024, 4 * in.size()), tcap = 1 << 16;
             for (;;) {
                 P.a.resize(cap * N); P.b.resize(cap * N);
                 ta.words.resize(tcap * N); ta.keys.resize(tcap * (N / 2));

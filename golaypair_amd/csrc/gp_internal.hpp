@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -11,6 +14,7 @@
 #include <vector>
 
 #include "gp_common.hpp"
+#include "gp_host.hpp"
 #include "golay_gpu.h"
 
 namespace gp {
@@ -75,10 +79,17 @@ struct TileDesc {
     int64_t first_row;
 };
 
-struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
-    void init() { GP_HIP(hipEventCreate(&a)); GP_HIP(hipEventCreate(&b)); }
-    ~Timer() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); }
+// Host wall-clock phases, printed to stderr when GOLAY_PROFILE is set.
+struct Phase {
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    explicit Phase(const char* n) : name(n), t0(std::chrono::steady_clock::now()) {}
+    ~Phase() {
+        static const bool on = std::getenv("GOLAY_PROFILE") != nullptr;
+        if (on)
+            std::fprintf(stderr, "[golay] %-28s %9.3f ms\n", name,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
 };
 
 }  // namespace gp
@@ -93,6 +104,11 @@ struct gp_ctx {
     gp::DevBuf words, tiles, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     int num_cus = 256;
+    // result of the last gp_uncompress that failed with GP_ERR_CAPACITY
+    bool unc_cache_valid = false;
+    uint64_t unc_cache_key = 0;
+    gp::PairSet unc_cache_out;
+    gp::CandSet unc_cache_ta, unc_cache_tb;
     // named scratch buffers of the pipeline stages
     std::map<std::string, std::unique_ptr<gp::DevBuf>> pool;
     gp::DevBuf& buf(const std::string& name) {

@@ -34,6 +34,7 @@ struct UncBatch {
     const int8_t* d_tup = nullptr;
     const uint64_t* d_pm1 = nullptr;
     const double* d_c64 = nullptr;
+    const int32_t* d_pos64 = nullptr;
     uint64_t* d_surv = nullptr;      // survivors: (table side << 40) | leaf
     int64_t nsurv = 0;
 };

@@ -14,6 +14,8 @@
 // FP64 keys are computed for the emitted pairs only (their output order is the
 // text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
 // of each emitted pair is checked (counted in tie_band when it fails).
+#include <thread>
+
 #include "unc_kernels.inc"  // opens namespace gp (closed at the end of this file)
 
 // ---------------------------------------------------------------------------
@@ -21,6 +23,7 @@
 
 __global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restrict__ sides, int nsides,
                                                          int64_t nitems, const int32_t* __restrict__ radix,
+                                                         const int32_t* __restrict__ pos32,
                                                          const float* __restrict__ c32, const float* __restrict__ mid,
                                                          const int8_t* __restrict__ ent, const float* __restrict__ w32,
                                                          UncCtl* ctl, uint64_t* __restrict__ surv, int64_t scap,
@@ -49,7 +52,7 @@ __global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restri
                 const int R = radix[S.radix_off + j];
                 const int d = (int)(rem % R);
                 rem /= R;
-                if (lane < nch) xo += c32[S.c32_off + ((int64_t)j * S.rmax + d) * nch + lane];
+                if (lane < nch) xo += c32[pos32[S.c32_off + j] + (int64_t)d * nch + lane];
             }
         }
         for (int64_t m = 0; m < S.p_mid; ++m) {
@@ -291,11 +294,104 @@ struct Tables {
     std::vector<uint4> sufb, frags;
     std::vector<double> c64;
     std::vector<uint64_t> pm1;
+    std::vector<int32_t> pos32, pos64;  // per side and position: block offsets into Shared
     int64_t items = 0;
 };
 
-void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const std::vector<double>& cs,
-              const std::vector<double>& sn, int group, bool fp32) {
+// Tables shared by every side of a call: the per-(position, letter) DFT
+// contribution blocks (FP32 channel layout and FP64 (Re, Im) pairs) and the
+// MFMA A fragments of each suffix start b.
+struct Shared {
+    std::map<std::pair<int, int>, int> block;  // (position j, letter at j) -> block id
+    std::vector<int32_t> off32, off64;
+    std::vector<float> c32;
+    std::vector<double> c64;
+    std::map<int, int64_t> frag_off;  // suffix start b -> offset into frags
+    std::map<int, double> rho;        // suffix start b -> max BF16 split residual
+    std::vector<uint4> frags;
+    int id(int j, int letter) const { return block.at({j, letter}); }
+};
+
+void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& cs, const std::vector<double>& sn) {
+    const auto key = std::make_pair(j, S.orig[j]);
+    if (SH.block.count(key)) return;
+    const int L = S.L, r = S.r, N = S.N, K = N / 2, mt = (K + 15) / 16;
+    const int64_t R = (int64_t)S.T[j].size();
+    SH.block[key] = (int)SH.off32.size();
+    SH.off32.push_back((int32_t)SH.c32.size());
+    SH.off64.push_back((int32_t)SH.c64.size());
+    SH.c32.resize(SH.c32.size() + (size_t)R * mt * 32, 0.f);
+    SH.c64.resize(SH.c64.size() + (size_t)R * 2 * K, 0.0);
+    float* c32 = &SH.c32[SH.off32.back()];
+    double* c64 = &SH.c64[SH.off64.back()];
+    for (int64_t d = 0; d < R; ++d)
+        for (int k = 1; k <= K; ++k) {
+            double re = 0, im = 0;
+            for (int i = 0; i < r; ++i) {
+                const int p = j + L * i;
+                const int mm = (int)(((int64_t)k * p) % N);
+                re += S.T[j][d][i] * cs[mm];
+                im += S.T[j][d][i] * sn[mm];
+            }
+            c64[(d * K + (k - 1)) * 2] = re;
+            c64[(d * K + (k - 1)) * 2 + 1] = im;
+            const int t = (k - 1) / 16, row = (k - 1) % 16;
+            c32[d * mt * 32 + t * 32 + row] = (float)re;
+            c32[d * mt * 32 + t * 32 + 16 + row] = (float)im;
+        }
+}
+
+void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
+    if (SH.frag_off.count(b)) return;
+    const int K = N / 2, mt = (K + 15) / 16, nent = (L - b) * r, nbs = std::max(1, (nent + 15) / 16);
+    SH.frag_off[b] = (int64_t)SH.frags.size();
+    double rho = 0;
+    for (int t = 0; t < mt; ++t)
+        for (int st = 0; st < 2 * nbs; ++st)
+            for (int lane = 0; lane < 64; ++lane) {
+                uint16_t hv[8];
+                for (int i8 = 0; i8 < 8; ++i8) {
+                    const int row = lane % 32;
+                    const int e = 16 * (st % nbs) + 8 * (lane / 32) + i8;
+                    const int part = st / nbs;
+                    const int k = 16 * t + row % 16 + 1;
+                    uint16_t v = 0;
+                    if (e < nent && k <= K) {
+                        const int j = b + e / r, i = e % r;
+                        const int p = j + L * i;
+                        const int mm = (int)(((int64_t)k * p) % N);
+                        const double w = row >= 16 ? sn[mm] : cs[mm];
+                        const uint16_t hi = bf16_rne_u((float)w);
+                        const double r1 = w - bf16_d(hi);
+                        const uint16_t lo = bf16_rne_u((float)r1);
+                        rho = std::max(rho, std::fabs(r1 - bf16_d(lo)));
+                        v = part ? lo : hi;
+                    }
+                    hv[i8] = v;
+                }
+                uint4 q;
+                q.x = hv[0] | ((uint32_t)hv[1] << 16);
+                q.y = hv[2] | ((uint32_t)hv[3] << 16);
+                q.z = hv[4] | ((uint32_t)hv[5] << 16);
+                q.w = hv[6] | ((uint32_t)hv[7] << 16);
+                SH.frags.push_back(q);
+            }
+    SH.rho[b] = rho;
+}
+
+// suffix start b and mid start a of a side (uncompression split of the leaf index)
+void side_split(const SideHost& S, int& a, int& b) {
+    const int L = S.L;
+    b = L;
+    int64_t ps = 1;
+    while (b > 0 && ps < 64) { --b; ps *= (int64_t)S.T[b].size(); }
+    a = b;
+    int64_t pm = 1;
+    while (a > 0 && pm < 16 && pm * (int64_t)S.T[a - 1].size() <= 4096) { --a; pm *= (int64_t)S.T[a].size(); }
+}
+
+void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int order, int paf,
+              const std::vector<double>& cs, const std::vector<double>& sn, int group, bool fp32) {
     UncSide D{};
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
@@ -305,13 +401,11 @@ void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const 
     std::vector<int64_t> R(L);
     for (int j = 0; j < L; ++j) R[j] = (int64_t)S.T[j].size();
     // split: suffix [b, L) with p_suf >= 64, mid [a, b) with 16 <= p_mid <= 4096
-    int b = L;
-    int64_t ps = 1;
-    while (b > 0 && ps < 64) { --b; ps *= R[b]; }
-    int a = b;
-    int64_t pm = 1;
-    while (a > 0 && pm < 16 && pm * R[a - 1] <= 4096) { --a; pm *= R[a]; }
-    int64_t po = 1;
+    int a, b;
+    side_split(S, a, b);
+    int64_t ps = 1, pm = 1, po = 1;
+    for (int j = b; j < L; ++j) ps *= R[j];
+    for (int j = a; j < b; ++j) pm *= R[j];
     for (int j = 0; j < a; ++j) po *= R[j];
     D.a = a; D.b = b; D.p_suf = ps; D.p_mid = pm; D.p_out = po; D.leaves = (int64_t)S.leaves;
     const int nent = (L - b) * r;
@@ -343,27 +437,14 @@ void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const 
                 T.pm1[D.pm1_off + (int64_t)j * rmax + d] = msk;
             }
     }
-    // FP64 per-position contributions and FP32 channel copies
-    D.c64_off = (int64_t)T.c64.size();
-    T.c64.resize(T.c64.size() + (size_t)L * rmax * 2 * K, 0.0);
-    D.c32_off = (int64_t)T.c32.size();
-    T.c32.resize(T.c32.size() + (size_t)L * rmax * mt * 32, 0.f);
-    for (int j = 0; j < L; ++j)
-        for (int64_t d = 0; d < R[j]; ++d)
-            for (int k = 1; k <= K; ++k) {
-                double re = 0, im = 0;
-                for (int i = 0; i < r; ++i) {
-                    const int p = j + L * i;
-                    const int mm = (int)(((int64_t)k * p) % N);
-                    re += S.T[j][d][i] * cs[mm];
-                    im += S.T[j][d][i] * sn[mm];
-                }
-                double* c = &T.c64[D.c64_off + (((int64_t)j * rmax + d) * K + (k - 1)) * 2];
-                c[0] = re; c[1] = im;
-                const int t = (k - 1) / 16, row = (k - 1) % 16;
-                T.c32[D.c32_off + ((int64_t)j * rmax + d) * mt * 32 + t * 32 + row] = (float)re;
-                T.c32[D.c32_off + ((int64_t)j * rmax + d) * mt * 32 + t * 32 + 16 + row] = (float)im;
-            }
+    // per-position contribution blocks (shared between sides)
+    D.c32_off = (int64_t)T.pos32.size();
+    D.c64_off = (int64_t)T.pos64.size();
+    for (int j = 0; j < L; ++j) {
+        const int id = SH.id(j, S.orig[j]);
+        T.pos32.push_back(SH.off32[id]);
+        T.pos64.push_back(SH.off64[id]);
+    }
     // mid table: FP64 sums over the middle digits, rounded once to FP32
     D.mid_off = (int64_t)T.mid.size();
     T.mid.resize(T.mid.size() + (size_t)pm * mt * 32, 0.f);
@@ -373,7 +454,7 @@ void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const 
         for (int64_t m = 0; m < pm; ++m) {
             std::fill(acc.begin(), acc.end(), 0.0);
             for (int j = a; j < b; ++j) {
-                const double* c = &T.c64[D.c64_off + ((int64_t)j * rmax + dg[j]) * K * 2];
+                const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])] + (int64_t)dg[j] * K * 2];
                 for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
             }
             for (int k = 1; k <= K; ++k) {
@@ -435,40 +516,9 @@ void add_side(Tables& T, const SideHost& S, int rmax, int order, int paf, const 
                 }
         }
     }
-    // A fragments: suffix DFT weights, BF16 hi and lo
-    D.frag_off = (int64_t)T.frags.size();
-    double rho = 0;
-    if (!fp32)
-        for (int t = 0; t < mt; ++t)
-            for (int st = 0; st < 2 * D.nbs; ++st)
-                for (int lane = 0; lane < 64; ++lane) {
-                    uint16_t hv[8];
-                    for (int i8 = 0; i8 < 8; ++i8) {
-                        const int row = lane % 32;
-                        const int e = 16 * (st % D.nbs) + 8 * (lane / 32) + i8;
-                        const int part = st / D.nbs;
-                        const int k = 16 * t + row % 16 + 1;
-                        uint16_t v = 0;
-                        if (e < nent && k <= K) {
-                            const int j = b + e / r, i = e % r;
-                            const int p = j + L * i;
-                            const int mm = (int)(((int64_t)k * p) % N);
-                            const double w = row >= 16 ? sn[mm] : cs[mm];
-                            const uint16_t hi = bf16_rne_u((float)w);
-                            const double r1 = w - bf16_d(hi);
-                            const uint16_t lo = bf16_rne_u((float)r1);
-                            rho = std::max(rho, std::fabs(r1 - bf16_d(lo)));
-                            v = part ? lo : hi;
-                        }
-                        hv[i8] = v;
-                    }
-                    uint4 q;
-                    q.x = hv[0] | ((uint32_t)hv[1] << 16);
-                    q.y = hv[2] | ((uint32_t)hv[3] << 16);
-                    q.z = hv[4] | ((uint32_t)hv[5] << 16);
-                    q.w = hv[6] | ((uint32_t)hv[7] << 16);
-                    T.frags.push_back(q);
-                }
+    // A fragments (suffix DFT weights, BF16 hi and lo), shared per suffix start
+    D.frag_off = fp32 ? 0 : SH.frag_off.at(b);
+    const double rho = fp32 ? 0.0 : SH.rho.at(b);
     // error bound of the filter's Re/Im
     double s1_pre = 0, s1_suf = 0;
     for (int j = 0; j < L; ++j) {
@@ -530,12 +580,61 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
     std::vector<double> sn(ctx->tables.trig64.begin() + td.trig_off + N,
                            ctx->tables.trig64.begin() + td.trig_off + 2 * N);
     Tables T;
+    Shared SH;
     uint64_t total_leaves = 0;
-    for (size_t i = 0; i < sides.size(); ++i) {
-        if (sides[i].leaves == 0) continue;
-        add_side(T, sides[i], rmax, order, paf, cs, sn, group[i], fp32);
-        B.side_of.push_back((int)i);
-        total_leaves += sides[i].leaves;
+    {
+        Phase ph("unc_filter: side tables");
+        std::vector<size_t> live;
+        for (size_t i = 0; i < sides.size(); ++i)
+            if (sides[i].leaves != 0) live.push_back(i);
+        // shared per-(position, letter) blocks and per-suffix-start fragments
+        for (size_t i : live) {
+            for (int j = 0; j < sides[i].L; ++j) add_block(SH, sides[i], j, cs, sn);
+            if (!fp32) {
+                int a, b;
+                side_split(sides[i], a, b);
+                add_frags(SH, b, sides[i].L, sides[i].r, N, cs, sn);
+            }
+        }
+        // per-side tables on host threads, then concatenated with rebased offsets
+        std::vector<Tables> part(live.size());
+        const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t q = t; q < live.size(); q += nt)
+                    add_side(part[q], SH, sides[live[q]], rmax, order, paf, cs, sn, group[live[q]], fp32);
+            });
+        for (auto& x : th) x.join();
+        for (size_t q = 0; q < live.size(); ++q) {
+            UncSide D = part[q].sides[0];
+            D.item_base = T.items;
+            D.radix_off += (int64_t)T.radix.size();
+            D.tup_off += (int64_t)T.tup.size();
+            D.c32_off += (int64_t)T.pos32.size();
+            D.mid_off += (int64_t)T.mid.size();
+            D.sufb_off += (int64_t)T.sufb.size();
+            D.c64_off += (int64_t)T.pos64.size();
+            D.ent_off += (int64_t)T.ent.size();
+            D.w32_off += (int64_t)T.w32.size();
+            if (D.pm1_off >= 0) D.pm1_off += (int64_t)T.pm1.size();
+            T.items += part[q].items;
+            auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+            app(T.radix, part[q].radix);
+            app(T.tup, part[q].tup);
+            app(T.pos32, part[q].pos32);
+            app(T.mid, part[q].mid);
+            app(T.sufb, part[q].sufb);
+            app(T.pos64, part[q].pos64);
+            app(T.ent, part[q].ent);
+            app(T.w32, part[q].w32);
+            app(T.pm1, part[q].pm1);
+            T.sides.push_back(D);
+            B.side_of.push_back((int)live[q]);
+            total_leaves += sides[live[q]].leaves;
+            Tables().sides.swap(part[q].sides);
+            part[q] = Tables();
+        }
     }
     B.N = N;
     B.K = N / 2;
@@ -546,11 +645,13 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
     B.d_radix = upload(ctx, "u_radix", T.radix);
     B.d_tup = upload(ctx, "u_tup", T.tup);
     B.d_pm1 = upload(ctx, "u_pm1", T.pm1);
-    B.d_c64 = upload(ctx, "u_c64", T.c64);
-    auto* d_c32 = upload(ctx, "u_c32", T.c32);
+    B.d_c64 = upload(ctx, "u_c64", SH.c64);
+    B.d_pos64 = upload(ctx, "u_pos64", T.pos64);
+    auto* d_pos32 = upload(ctx, "u_pos32", T.pos32);
+    auto* d_c32 = upload(ctx, "u_c32", SH.c32);
     auto* d_mid = upload(ctx, "u_mid", T.mid);
     auto* d_sufb = upload(ctx, "u_sufb", T.sufb);
-    auto* d_frags = upload(ctx, "u_frags", T.frags);
+    auto* d_frags = upload(ctx, "u_frags", SH.frags);
     auto* d_ent = upload(ctx, "u_ent", T.ent);
     auto* d_w32 = upload(ctx, "u_w32", T.w32);
     std::vector<int64_t> toff{td.trig_off};
@@ -570,10 +671,10 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
         const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
         if (!fp32)
             hipLaunchKernelGGL(k_unc_filter, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items, B.d_radix,
-                               d_c32, d_mid, d_sufb, d_frags, ctl, d_surv, scap, d_band, bcap);
+                               d_pos32, d_c32, d_mid, d_sufb, d_frags, ctl, d_surv, scap, d_band, bcap);
         else
             hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items,
-                               B.d_radix, d_c32, d_mid, d_ent, d_w32, ctl, d_surv, scap, d_band, bcap);
+                               B.d_radix, d_pos32, d_c32, d_mid, d_ent, d_w32, ctl, d_surv, scap, d_band, bcap);
         GP_HIP(hipGetLastError());
         GP_HIP(hipEventRecord(ctx->ev1, st));
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
@@ -707,7 +808,7 @@ CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side) {
     int8_t* w = ctx->dev<int8_t>("u_cw", (size_t)m * B.N);
     uint8_t* k = ctx->dev<uint8_t>("u_ck", (size_t)m * B.K);
     UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
-    hipLaunchKernelGGL(k_unc_keys, dim3(ctx->num_cus * 4), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_c64,
+    hipLaunchKernelGGL(k_unc_keys, dim3(ctx->num_cus * 4), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pos64, B.d_c64,
                        sel, (int64_t)m, B.bound, w, k, ctl);
     GP_HIP(hipGetLastError());
     c.words.resize((size_t)m * B.N);
