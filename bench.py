@@ -14,6 +14,7 @@ path); the plain FP32 path and the CPU oracle are reported beside it.
 import argparse
 import json
 import os
+import re
 import sys
 import threading
 import time
@@ -28,19 +29,45 @@ FLOPS_PER_LEAF = 4 * 45 * 22 + 3 * 22       # algorithmic: real DFT k=1..22 of n
 MFMA_PEAK_TFLOPS = 2500.0                    # MI355X dense BF16
 
 
+STAGE_MS = {}
+
+
 def search_step(ctx, g):
     st = g.GpStats()
+    t = [time.perf_counter()]
     a, b, s = ctx.generate(ORDER, C0, PAF)
     st.add(s)
+    t.append(time.perf_counter())
     sa = ctx.sort_candidates(a)
     sb = ctx.sort_candidates(b)
+    t.append(time.perf_counter())
     rc, pa, pb, s = ctx.match(ORDER, C0, PAF, sa, sb)
     st.add(s)
+    t.append(time.perf_counter())
     ua, ub, _, _, su = ctx.uncompress(ORDER, C0, C1, PAF, pa, pb)
     st.add(su)
+    t.append(time.perf_counter())
     fa, fb, s = ctx.filter(ORDER, C1, ua, ub)
     st.add(s)
+    t.append(time.perf_counter())
+    for i, name in enumerate(("gen", "sort", "match", "uncompress", "filter")):
+        STAGE_MS[name] = (t[i + 1] - t[i]) * 1e3
     return st, su, len(pa), len(ua), len(fa)
+
+
+def profiled_traffic(kernel="k_unc_filter"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (KiB per dispatch, scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "k_unc_filter_v45_legendre_search.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic %s fetch_kib=([\d.]+) fetch_scale=([\d.]+) write_kib=([\d.]+) write_scale=([\d.]+)"
+                     % kernel, line)
+        if m:
+            f, fs, w, ws = map(float, m.groups())
+            return (f * fs + w * ws) * 1024.0
+    return None
 
 
 def timed(ctx, g, steps, warmup):
@@ -97,6 +124,18 @@ def microbench(ctx, g, mode):
             "accepted": int(acc.sum()), "digest": "%016x" % digest, "tie_band": st.tie_band}
 
 
+def copy_probe(mib=1024, reps=4):
+    """A plain device-to-device copy of known size (PMC calibration for FETCH_SIZE
+    and WRITE_SIZE): `reps` copies of `mib` MiB."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": mib << 20, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +144,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB copy (PMC calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the search runs on a single MI355X (no multi-GPU path is built); "
               "run with --gpus 1", file=sys.stderr)
@@ -135,13 +178,14 @@ def main():
                    "order": ORDER, "compress": [C0, C1], "paf": PAF, "psd": "split-BF16 MFMA, FP32 accumulate, "
                    "FP64 band recheck and keys"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": profiled_traffic(),
                      "kernel": "k_unc_filter", "kernel_ms_per_launch": kern_ms,
                      "flops_per_candidate": FLOPS_PER_LEAF, "candidates_per_launch": su.psd_decisions / launches},
         "end_to_end_s": sec,
+        "stage_ms_last_step": dict(STAGE_MS),
         "search": {"pairs_c3": npairs3, "pairs_c1": npairs1, "unique": nunique, "psd_decisions": st.psd_decisions,
                    "leaves": su.psd_decisions, "band_rechecks": st.band_rechecks, "tie_band": st.tie_band,
-                   "pair_checks": st.pair_checks, "kernel_ms": st.kernel_ms},
+                   "psd_pass": st.psd_pass, "pair_checks": st.pair_checks, "kernel_ms": st.kernel_ms},
     }
     if not args.no_fp32:
         ctx.set_mode(g.GP_PSD_FP32)
