@@ -15,7 +15,8 @@
 // Walking the pairs in (a, b) order and erasing each survivor's class
 // (filter.cpp:117-144) therefore keeps the lexicographically smallest input
 // member of every orbit.  Here every pair's orbit minimum is computed on the GPU
-// (one pair per lane); pairs are then grouped by orbit minimum on the host.
+// (one pair per wave, as an order-preserving packed key); pairs are then grouped
+// by orbit minimum on the host.
 #include <algorithm>
 #include <cstring>
 #include <functional>
@@ -31,81 +32,157 @@ struct FilterParams {
     int32_t alt;
     int32_t nunits;
     int32_t units[64];
+    int32_t mx, sh, B, E;  // value v packs as (v + mx) >> sh in B bits, E = 64 / B values per word
 };
 
-// lexicographic compare of transform image against best[]: -1 less, 0 equal, 1 greater
-__device__ __forceinline__ int cmp_img(const int8_t* X, int n, int u, int t, int kappa, int sg, const int8_t* best) {
-    int idx = t;
-    for (int i = 0; i < n; ++i) {
-        int v = X[idx] * sg;
-        if (kappa && (i & 1)) v = -v;
-        if (v != best[i]) return v < best[i] ? -1 : 1;
-        idx += u;
-        if (idx >= n) idx -= n;
-    }
-    return 0;
-}
-
-__device__ __forceinline__ void write_img(const int8_t* X, int n, int u, int t, int kappa, int sg, int8_t* out) {
-    int idx = t;
-    for (int i = 0; i < n; ++i) {
-        int v = X[idx] * sg;
-        if (kappa && (i & 1)) v = -v;
-        out[i] = (int8_t)v;
-        idx += u;
-        if (idx >= n) idx -= n;
-    }
-}
-
-__global__ void k_orbit_min(FilterParams P, const int8_t* __restrict__ pa, const int8_t* __restrict__ pb, int64_t m,
-                            int8_t* __restrict__ canon) {
+// Packed key of the image s * (-1)^(kappa i) * X[(u i + t) mod n], i = 0..n-1:
+// B bits per value, first value in the most significant bits of word 0, so
+// that unsigned word-by-word order is the signed lexicographic order of images.
+template <int W>
+__device__ __forceinline__ void img_key(const int8_t* X, const FilterParams& P, int u, int t, int kappa, int sg,
+                                        uint64_t (&k)[W]) {
     const int n = P.n;
-    int8_t besta[kMaxN], bestb[kMaxN];
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
-        const int8_t* a = pa + q * n;
-        const int8_t* b = pb + q * n;
-        const int nk = P.alt ? 2 : 1, ns = P.alt ? 2 : 1;
-        // phase 1: minimum a-part
-        bool have = false;
-        for (int sw = 0; sw < 2; ++sw) {
-            const int8_t* X = sw ? b : a;
-            for (int ui = 0; ui < P.nunits; ++ui)
-                for (int kappa = 0; kappa < nk; ++kappa)
-                    for (int si = 0; si < ns; ++si)
-                        for (int t = 0; t < n; ++t) {
-                            const int sg = si ? -1 : 1;
-                            if (!have || cmp_img(X, n, P.units[ui], t, kappa, sg, besta) < 0) {
-                                write_img(X, n, P.units[ui], t, kappa, sg, besta);
-                                have = true;
-                            }
-                        }
+    int idx = t, i = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t acc = 0;
+        const int iend = min(n, (w + 1) * P.E);
+        const int cnt = max(0, iend - i);
+        for (; i < iend; ++i) {
+            int v = X[idx] * sg;
+            if (kappa & i & 1) v = -v;
+            acc = (acc << P.B) | (uint64_t)((v + P.mx) >> P.sh);
+            idx += u;
+            if (idx >= n) idx -= n;
         }
-        // phase 2: minimum b-part over the transforms reaching the minimum a-part
-        bool haveb = false;
-        for (int sw = 0; sw < 2; ++sw) {
-            const int8_t* X = sw ? b : a;
-            const int8_t* Y = sw ? a : b;
-            for (int ui = 0; ui < P.nunits; ++ui)
-                for (int kappa = 0; kappa < nk; ++kappa)
-                    for (int si = 0; si < ns; ++si)
-                        for (int t = 0; t < n; ++t) {
-                            const int u = P.units[ui];
-                            if (cmp_img(X, n, u, t, kappa, si ? -1 : 1, besta) != 0) continue;
-                            for (int e = 0; e < 2; ++e) {
-                                const int ub = e ? n - u : u;
-                                for (int sb = 0; sb < ns; ++sb)
-                                    for (int tb = 0; tb < n; ++tb) {
-                                        const int sgb = sb ? -1 : 1;
-                                        if (!haveb || cmp_img(Y, n, ub, tb, kappa, sgb, bestb) < 0) {
-                                            write_img(Y, n, ub, tb, kappa, sgb, bestb);
-                                            haveb = true;
-                                        }
-                                    }
-                            }
-                        }
+        k[w] = cnt ? acc << (P.B * (P.E - cnt)) : 0ull;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ bool key_less(const uint64_t (&x)[W], const uint64_t (&y)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (x[w] != y[w]) return x[w] < y[w];
+    return false;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_eq(const uint64_t (&x)[W], const uint64_t (&y)[W]) {
+    bool e = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) e = e && x[w] == y[w];
+    return e;
+}
+
+// minimum over the 64 lanes of the wave, returned to every lane
+template <int W>
+__device__ __forceinline__ void wave_min(uint64_t (&k)[W]) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        uint64_t o[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) o[w] = __shfl_xor(k[w], off);
+        if (key_less(o, k))
+#pragma unroll
+            for (int w = 0; w < W; ++w) k[w] = o[w];
+    }
+}
+
+// One pair per wave.  Phase 1: the lanes share the a-part transforms
+// (swap, unit, kappa, sign, shift) and reduce the minimum a-part key.  Phase 2:
+// every transform reaching that minimum (found by a ballot over the same
+// enumeration) is broadcast, and the lanes share its b-part transforms
+// (e, sign, shift) for the minimum b-part key.  canon[q] = (a key, b key).
+template <int W>
+__global__ void __launch_bounds__(256) k_orbit_min(FilterParams P, const int8_t* __restrict__ pa,
+                                                   const int8_t* __restrict__ pb, int64_t m,
+                                                   uint64_t* __restrict__ canon) {
+    __shared__ int8_t sx[4][2][kMaxN];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = P.n;
+    const int nk = P.alt ? 2 : 1, ns = P.alt ? 2 : 1;
+    const int ncand = 2 * P.nunits * nk * ns * n;
+    const int nbimg = 2 * ns * n;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + wv; q < m; q += (int64_t)gridDim.x * 4) {
+        if (lane < n) {
+            sx[wv][0][lane] = pa[q * n + lane];
+            sx[wv][1][lane] = pb[q * n + lane];
         }
-        int8_t* o = canon + q * 2 * n;
-        for (int i = 0; i < n; ++i) { o[i] = besta[i]; o[n + i] = bestb[i]; }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        auto decode = [&](int c, int& sw, int& u, int& kappa, int& sg, int& t) {
+            t = c % n;
+            c /= n;
+            sg = (c % ns) ? -1 : 1;
+            c /= ns;
+            kappa = c % nk;
+            c /= nk;
+            u = P.units[c % P.nunits];
+            sw = c / P.nunits;
+        };
+        uint64_t ka[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) ka[w] = ~0ull;
+        for (int c = lane; c < ncand; c += 64) {
+            int sw, u, kappa, sg, t;
+            decode(c, sw, u, kappa, sg, t);
+            uint64_t k[W];
+            img_key<W>(sx[wv][sw], P, u, t, kappa, sg, k);
+            if (key_less(k, ka))
+#pragma unroll
+                for (int w = 0; w < W; ++w) ka[w] = k[w];
+        }
+        uint64_t lmin[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) lmin[w] = ka[w];
+        wave_min<W>(ka);
+        // only a lane whose own minimum is the wave minimum holds transforms reaching it
+        const bool mine = key_eq(lmin, ka);
+        uint64_t kb[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) kb[w] = ~0ull;
+        {
+            for (int c0 = 0; c0 < ncand; c0 += 64) {
+                const int c = c0 + lane;
+                bool hit = false;
+                if (mine && c < ncand) {
+                    int sw, u, kappa, sg, t;
+                    decode(c, sw, u, kappa, sg, t);
+                    uint64_t k[W];
+                    img_key<W>(sx[wv][sw], P, u, t, kappa, sg, k);
+                    hit = key_eq(k, ka);
+                }
+                unsigned long long hits = __ballot(hit);
+                while (hits) {
+                    const int bit = __builtin_ctzll(hits);
+                    hits &= hits - 1;
+                    int sw, u, kappa, sg, t;
+                    decode(c0 + bit, sw, u, kappa, sg, t);
+                    const int8_t* Y = sx[wv][sw ? 0 : 1];
+                    for (int j = lane; j < nbimg; j += 64) {
+                        const int tb = j % n;
+                        const int sgb = ((j / n) % ns) ? -1 : 1;
+                        const int e = j / (n * ns);
+                        const int ub = e ? (n - u) % n : u;
+                        uint64_t k[W];
+                        img_key<W>(Y, P, n == 1 ? 0 : ub, tb, kappa, sgb, k);
+                        if (key_less(k, kb))
+#pragma unroll
+                            for (int w = 0; w < W; ++w) kb[w] = k[w];
+                    }
+                }
+            }
+        }
+        wave_min<W>(kb);
+        if (lane == 0) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                canon[q * 2 * W + w] = ka[w];
+                canon[q * 2 * W + W + w] = kb[w];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -124,19 +201,38 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     for (int u = 1; u < n || (n == 1 && u == 1); ++u)
         if (std::gcd(u, n) == 1) P.units[P.nunits++] = u;
     if (n == 1) { P.nunits = 1; P.units[0] = 0; }
+    // key packing: the values and their negations lie in [-mx, mx]; halve when all have mx's parity
+    int mx = 0;
+    bool same_parity = true;
+    for (int64_t i = 0; i < m * n; ++i) {
+        mx = std::max(mx, std::abs((int)in.a[i]));
+        mx = std::max(mx, std::abs((int)in.b[i]));
+    }
+    for (int64_t i = 0; i < m * n && same_parity; ++i)
+        same_parity = ((in.a[i] + mx) % 2 == 0) && ((in.b[i] + mx) % 2 == 0);
+    P.mx = mx;
+    P.sh = same_parity ? 1 : 0;
+    const int range = (2 * mx) >> P.sh;
+    P.B = range <= 1 ? 1 : range <= 3 ? 2 : range <= 15 ? 4 : 8;
+    P.E = 64 / P.B;
+    const int wneed = (n + P.E - 1) / P.E;
+    const int W = wneed <= 1 ? 1 : wneed <= 2 ? 2 : wneed <= 4 ? 4 : 8;
     hipStream_t st = ctx->stream;
     int8_t* da = ctx->dev<int8_t>("f_a", (size_t)m * n);
     int8_t* db = ctx->dev<int8_t>("f_b", (size_t)m * n);
-    int8_t* dc = ctx->dev<int8_t>("f_c", (size_t)m * 2 * n);
+    uint64_t* dc = ctx->dev<uint64_t>("f_c", (size_t)m * 2 * W);
     GP_HIP(hipMemcpyAsync(da, in.a.data(), (size_t)m * n, hipMemcpyHostToDevice, st));
     GP_HIP(hipMemcpyAsync(db, in.b.data(), (size_t)m * n, hipMemcpyHostToDevice, st));
     GP_HIP(hipEventRecord(ctx->ev0, st));
-    hipLaunchKernelGGL(k_orbit_min, dim3((unsigned)std::min<int64_t>((m + 63) / 64, ctx->num_cus * 8)), dim3(64), 0,
-                       st, P, da, db, m, dc);
+    const dim3 grid((unsigned)std::min<int64_t>((m + 3) / 4, (int64_t)ctx->num_cus * 16));
+    if (W == 1) hipLaunchKernelGGL(k_orbit_min<1>, grid, dim3(256), 0, st, P, da, db, m, dc);
+    else if (W == 2) hipLaunchKernelGGL(k_orbit_min<2>, grid, dim3(256), 0, st, P, da, db, m, dc);
+    else if (W == 4) hipLaunchKernelGGL(k_orbit_min<4>, grid, dim3(256), 0, st, P, da, db, m, dc);
+    else hipLaunchKernelGGL(k_orbit_min<8>, grid, dim3(256), 0, st, P, da, db, m, dc);
     GP_HIP(hipGetLastError());
     GP_HIP(hipEventRecord(ctx->ev1, st));
-    std::vector<int8_t> canon((size_t)m * 2 * n);
-    GP_HIP(hipMemcpyAsync(canon.data(), dc, canon.size(), hipMemcpyDeviceToHost, st));
+    std::vector<uint64_t> canon((size_t)m * 2 * W);
+    GP_HIP(hipMemcpyAsync(canon.data(), dc, canon.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     GP_HIP(hipStreamSynchronize(st));
     float ms = 0;
     GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -150,8 +246,8 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
         return false;
     };
     auto canon_cmp = [&](int64_t x, int64_t y) {
-        for (int i = 0; i < 2 * n; ++i)
-            if (canon[x * 2 * n + i] != canon[y * 2 * n + i]) return canon[x * 2 * n + i] < canon[y * 2 * n + i] ? -1 : 1;
+        for (int w = 0; w < 2 * W; ++w)
+            if (canon[x * 2 * W + w] != canon[y * 2 * W + w]) return canon[x * 2 * W + w] < canon[y * 2 * W + w] ? -1 : 1;
         return 0;
     };
     std::vector<int64_t> idx(m);

@@ -72,3 +72,43 @@ def test_filter_random_orbits_vs_oracle(gpu_ctx, n, compress):
         ref = open(os.path.join(wd, "%d-unique" % order)).read()
     oa, ob = _gpu_filter(gpu_ctx, order, compress, a, b)
     assert _pairs_text(oa, ob) == ref
+
+
+def _random_map(rng, n, alt):
+    units = [u for u in range(1, max(n, 2)) if np.gcd(u, n) == 1] or [0]
+    return (int(rng.integers(2)), int(rng.choice(units)), int(rng.choice([1, -1])),
+            int(rng.integers(2)) if alt else 0, int(rng.choice([1, -1])) if alt else 1,
+            int(rng.choice([1, -1])) if alt else 1, int(rng.integers(n)), int(rng.integers(n)))
+
+
+# (n, compress, letters): odd and even alphabets, packed keys of 1..8 words, and
+# symmetric pairs whose a-part minimum is reached by many transforms
+@pytest.mark.parametrize("n,compress,letters", [
+    (45, 1, (-1, 1)), (20, 1, (-1, 1)), (15, 3, (-3, -1, 1, 3)), (18, 5, (-5, -3, -1, 1, 3, 5)),
+    (12, 2, (-2, 0, 2)), (20, 10, tuple(range(-10, 11, 2))), (40, 4, (-4, -2, 0, 2, 4)),
+    (60, 20, tuple(range(-20, 21, 2)))])
+def test_filter_alphabets_vs_oracle(gpu_ctx, n, compress, letters):
+    rng = np.random.default_rng(7 * n + compress)
+    alt = compress == 1 and n % 2 == 0
+    pairs = []
+    for k in range(6):
+        if k == 0:
+            p = (tuple([letters[-1]] * n), tuple([letters[0]] * n))
+        elif k == 1:
+            p = (tuple(letters[i % 2] for i in range(n)), tuple(letters[(i // 2) % len(letters)] for i in range(n)))
+        else:
+            p = (tuple(int(x) for x in rng.choice(letters, n)), tuple(int(x) for x in rng.choice(letters, n)))
+        for _ in range(4):
+            pairs.append(apply(_random_map(rng, n, alt), *p))
+    a = np.array([p[0] for p in pairs], np.int8)
+    b = np.array([p[1] for p in pairs], np.int8)
+    order = n * compress
+    with tempfile.TemporaryDirectory() as d:
+        wd = os.path.join(d, "order-%d" % order)
+        os.makedirs(wd)
+        open(os.path.join(wd, "%d-pairs-%d" % (order, compress)), "w").write(_pairs_text(a, b))
+        subprocess.run([_oracle.CLI, str(order), "-c", str(compress), "-d", d, "filter"], check=True,
+                       capture_output=True, timeout=600)
+        ref = open(os.path.join(wd, "%d-unique" % order)).read()
+    oa, ob = _gpu_filter(gpu_ctx, order, compress, a, b)
+    assert _pairs_text(oa, ob) == ref
