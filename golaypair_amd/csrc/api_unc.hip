@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -25,6 +26,7 @@ constexpr uint64_t kLeafBudget = 1ull << 30;  // leaves per device batch
 void build_sides(const PairSet& inp, int L, int c, int c2, int64_t q0, int64_t q1, std::vector<SideHost>& out) {
     const int64_t n = q1 - q0;
     out.resize(2 * n);
+    const auto tab = make_splits(c, c2);
     const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
     std::vector<std::thread> th;
     std::vector<std::string> err(nt);
@@ -32,8 +34,8 @@ void build_sides(const PairSet& inp, int L, int c, int c2, int64_t q0, int64_t q
         th.emplace_back([&, t] {
             try {
                 for (int64_t i = t; i < n; i += nt) {
-                    out[2 * i] = build_side(&inp.a[(q0 + i) * L], L, c, c2, 0);
-                    out[2 * i + 1] = build_side(&inp.b[(q0 + i) * L], L, c, c2, 1);
+                    out[2 * i] = build_side(tab, &inp.a[(q0 + i) * L], L, 0);
+                    out[2 * i + 1] = build_side(tab, &inp.b[(q0 + i) * L], L, 1);
                 }
             } catch (const std::exception& e) {
                 err[t] = e.what();
@@ -58,8 +60,8 @@ int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new
         SideHost S = build_side(orig, len, compress, new_compress, seqflag);
         for (int i = 0; i < len; ++i) orig[i] = (int8_t)S.orig[i];  // the reference rotates in place
         const int order = len * compress;
-        std::vector<SideHost> v{S};
-        UncBatch B = unc_filter(ctx, v, {0}, order, paf, stats);
+        const int g0 = 0;
+        UncBatch B = unc_filter(ctx, std::span<const SideHost>(&S, 1), std::span<const int>(&g0, 1), order, paf, stats);
         CandSet c = unc_side_cands(ctx, B, 0);
         c.n = S.N;
         c.nkey = S.N / 2;
@@ -77,6 +79,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         if (new_compress <= 0 || compress % new_compress) throw ArgError("gp_uncompress: bad compression pair");
         const int L = order / compress;
         if (in->count > 0 && in->n != L) throw ArgError("gp_uncompress: pair length != order/compress");
+        Phase ph_all("uncompress: total");
         PairSet inp = import_pairs(in);
         const int64_t P = inp.size();
         const int N = order / new_compress;
@@ -106,8 +109,10 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             Phase ph("uncompress: build sides");
             build_sides(inp, L, compress, new_compress, 0, P, all);
         }
-        int64_t q0 = 0;
-        while (q0 < P) {
+        // batches by leaf budget; the host tables of batch i+1 are prepared on
+        // another thread while the device runs batch i
+        std::vector<std::pair<int64_t, int64_t>> batches;
+        for (int64_t q0 = 0; q0 < P;) {
             uint64_t leaves = 0;
             int64_t q1 = q0;
             while (q1 < P) {
@@ -116,13 +121,30 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 leaves += add;
                 ++q1;
             }
-            std::vector<SideHost> sides(all.begin() + 2 * q0, all.begin() + 2 * q1);
-            std::vector<int> group(sides.size());
-            for (size_t i = 0; i < sides.size(); ++i) group[i] = (int)(i / 2);
+            batches.emplace_back(q0, q1);
+            q0 = q1;
+        }
+        std::vector<int> group(all.size());
+        for (size_t i = 0; i < all.size(); ++i) group[i] = (int)(i / 2);
+        UncTrig trig;
+        if (P > 0) trig = unc_trig(ctx, N);
+        const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
+        auto prepare = [&](size_t i) {
+            const auto [q0, q1] = batches[i];
+            const std::span<const SideHost> sides(all.data() + 2 * q0, (size_t)(2 * (q1 - q0)));
+            const std::span<const int> grp(group.data(), sides.size());
+            return unc_prepare(sides, grp, order, paf, fp32, trig);
+        };
+        std::future<std::shared_ptr<UncPrep>> next;
+        if (!batches.empty()) next = std::async(std::launch::async, prepare, 0);
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            const auto [q0, q1] = batches[bi];
+            std::shared_ptr<UncPrep> prep = next.get();
+            if (bi + 1 < batches.size()) next = std::async(std::launch::async, prepare, bi + 1);
             UncBatch B;
             {
                 Phase ph("uncompress: filter batch");
-                B = unc_filter(ctx, sides, group, order, paf, stats);
+                B = unc_launch(ctx, *prep, stats);
             }
             UncEmitted E;
             {
@@ -159,7 +181,6 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 result.b.insert(result.b.end(), &E.words[(E.count + i) * N], &E.words[(E.count + i) * N] + N);
             }
             if (stats) stats->pairs += ord.size();
-            q0 = q1;
         }
         int rc = GP_OK;
         if (P > 0) {

@@ -214,23 +214,29 @@ __global__ void k_gen_unpack(const u128* __restrict__ surv_w, int64_t m, int n, 
     }
 }
 
-// FP64 keys of accepted rows: A plain rint(PSD_k), B inverted bound - rint(PSD_k).
-__global__ void k_keys_fp64(const int8_t* __restrict__ words, int stride, int64_t m, int n,
-                            const double* __restrict__ trig, const uint8_t* __restrict__ decision,
-                            const uint8_t* __restrict__ side, int bound, uint8_t* __restrict__ keys,
-                            unsigned int* __restrict__ ties) {
+// Accepted rows, compacted: word (n bytes), FP64 key (K bytes: A plain rint(PSD_k),
+// B inverted bound - rint(PSD_k)) and side, each at its own slot (unordered).
+__global__ void k_gen_emit(const int8_t* __restrict__ words, int stride, int64_t m, int n,
+                           const double* __restrict__ trig, const uint8_t* __restrict__ decision,
+                           const uint8_t* __restrict__ side, int bound, unsigned long long* __restrict__ count,
+                           int8_t* __restrict__ ow, uint8_t* __restrict__ okeys, uint8_t* __restrict__ oside,
+                           unsigned int* __restrict__ ties) {
     const int K = n >> 1;
     double psd[kMaxK];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         if (decision[i] != kAccept) continue;
-        fp64_psd(words + i * stride, n, trig, trig + n, psd);
+        const int8_t* w = words + i * stride;
+        fp64_psd(w, n, trig, trig + n, psd);
+        const unsigned long long slot = atomicAdd(count, 1ull);  // slot < m: one per accepted row
         unsigned int t = 0;
         for (int k = 0; k < K; ++k) {
             const double f = psd[k] - floor(psd[k]);
             if (fabs(f - 0.5) < 1e-9) ++t;
             const int v = (int)rint(psd[k]);
-            keys[i * kMaxK + k] = (uint8_t)(side[i] ? bound - v : v);
+            okeys[slot * K + k] = (uint8_t)(side[i] ? bound - v : v);
         }
+        for (int j = 0; j < n; ++j) ow[slot * n + j] = w[j];
+        oside[slot] = side[i];
         if (t) atomicAdd(ties, t);
     }
 }
@@ -363,8 +369,9 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     const int stride = 16 * d.nb;
     int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)std::max<int64_t>(m, 1) * stride));
     uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)std::max<int64_t>(m, 1)));
-    std::vector<int8_t> words_h;
-    std::vector<uint8_t> keys_h, dec_h, side_h;
+    std::vector<int8_t> words_h;   // accepted rows only: [acc_n][n]
+    std::vector<uint8_t> keys_h, side_h;
+    int64_t acc_n = 0;
     if (m > 0) {
         hipLaunchKernelGGL(k_gen_unpack, dim3(ctx->num_cus * 4), dim3(256), 0, st, ctx->buf("g_w").as<u128>(), m, n, a0,
                            stride, d_words);
@@ -375,37 +382,44 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         tt[table] = make_thr(2.0 * order - paf + 0.001, E);
         std::vector<BatchSeg> bs{BatchSeg{table, 0, 0, m}};
         run_psd_filter(ctx, d_words, stride, m, bs, tt, d_dec, stats);
-        uint8_t* d_keys = static_cast<uint8_t*>(ctx->buf("g_keys").ensure((size_t)m * kMaxK + 16));
+        const int K = n / 2;
+        int8_t* d_ow = ctx->dev<int8_t>("g_ow", (size_t)m * n);
+        uint8_t* d_ok = ctx->dev<uint8_t>("g_keys", (size_t)m * std::max(K, 1));
+        uint8_t* d_os = ctx->dev<uint8_t>("g_os", (size_t)m);
+        auto* d_cnt = ctx->dev<unsigned long long>("g_emit_count", 1);
         unsigned int* d_ties = ctx->dev<unsigned int>("g_ties", 1);
         GP_HIP(hipMemsetAsync(d_ties, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(k_keys_fp64, dim3(ctx->num_cus * 4), dim3(256), 0, st, d_words, stride, m, n,
+        GP_HIP(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_gen_emit, dim3(ctx->num_cus * 4), dim3(256), 0, st, d_words, stride, m, n,
                            ctx->tables.d_trig64.as<double>() + d.trig_off, d_dec, ctx->buf("g_side").as<uint8_t>(), bound,
-                           d_keys, d_ties);
+                           d_cnt, d_ow, d_ok, d_os, d_ties);
         GP_HIP(hipGetLastError());
-        words_h.resize((size_t)m * stride);
-        keys_h.resize((size_t)m * kMaxK);
-        dec_h.resize((size_t)m);
-        side_h.resize((size_t)m);
+        unsigned long long cnt_h = 0;
         unsigned int ties = 0;
-        GP_HIP(hipMemcpyAsync(words_h.data(), d_words, words_h.size(), hipMemcpyDeviceToHost, st));
-        GP_HIP(hipMemcpyAsync(keys_h.data(), d_keys, keys_h.size(), hipMemcpyDeviceToHost, st));
-        GP_HIP(hipMemcpyAsync(dec_h.data(), d_dec, dec_h.size(), hipMemcpyDeviceToHost, st));
-        GP_HIP(hipMemcpyAsync(side_h.data(), ctx->buf("g_side").p, side_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(&cnt_h, d_cnt, sizeof cnt_h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipMemcpyAsync(&ties, d_ties, sizeof ties, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
+        if ((int64_t)cnt_h > m) throw std::runtime_error("gen: emitted more rows than candidates");
+        acc_n = (int64_t)cnt_h;
+        words_h.resize((size_t)acc_n * n);
+        keys_h.resize((size_t)acc_n * K);
+        side_h.resize((size_t)acc_n);
+        if (acc_n > 0) {
+            GP_HIP(hipMemcpyAsync(words_h.data(), d_ow, words_h.size(), hipMemcpyDeviceToHost, st));
+            GP_HIP(hipMemcpyAsync(keys_h.data(), d_ok, keys_h.size(), hipMemcpyDeviceToHost, st));
+            GP_HIP(hipMemcpyAsync(side_h.data(), d_os, side_h.size(), hipMemcpyDeviceToHost, st));
+            GP_HIP(hipStreamSynchronize(st));
+        }
         if (stats) stats->tie_band += ties;
     }
     // collect A and B in the reference's file order (lexicographic order of the words)
     const int K = n / 2;
     std::vector<int64_t> ia, ib;
-    for (int64_t i = 0; i < m; ++i) {
-        if (dec_h[i] != kAccept) continue;
-        (side_h[i] ? ib : ia).push_back(i);
-    }
+    for (int64_t i = 0; i < acc_n; ++i) (side_h[i] ? ib : ia).push_back(i);
     if (stats) stats->psd_pass += ia.size() + ib.size();
     auto lexless = [&](int64_t x, int64_t y) {
-        return std::lexicographical_compare(&words_h[x * stride], &words_h[x * stride] + n, &words_h[y * stride],
-                                            &words_h[y * stride] + n);
+        return std::lexicographical_compare(&words_h[x * n], &words_h[x * n] + n, &words_h[y * n],
+                                            &words_h[y * n] + n);
     };
     std::sort(ia.begin(), ia.end(), lexless);
     std::sort(ib.begin(), ib.end(), lexless);
@@ -415,8 +429,8 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         cs.words.resize(idx.size() * n);
         cs.keys.resize(idx.size() * K);
         for (size_t r = 0; r < idx.size(); ++r) {
-            std::memcpy(&cs.words[r * n], &words_h[idx[r] * stride], n);
-            std::memcpy(&cs.keys[r * K], &keys_h[idx[r] * kMaxK], K);
+            std::memcpy(&cs.words[r * n], &words_h[idx[r] * n], n);
+            std::memcpy(&cs.keys[r * K], &keys_h[idx[r] * K], K);
         }
     };
     emit(ia, out.a);

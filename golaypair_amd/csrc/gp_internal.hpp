@@ -84,7 +84,14 @@ struct Phase {
     const char* name;
     std::chrono::steady_clock::time_point t0;
     explicit Phase(const char* n) : name(n), t0(std::chrono::steady_clock::now()) {}
-    ~Phase() {
+    ~Phase() { report(); }
+    // ends this phase and starts the next one
+    void next(const char* n) {
+        report();
+        name = n;
+        t0 = std::chrono::steady_clock::now();
+    }
+    void report() const {
         static const bool on = std::getenv("GOLAY_PROFILE") != nullptr;
         if (on)
             std::fprintf(stderr, "[golay] %-28s %9.3f ms\n", name,

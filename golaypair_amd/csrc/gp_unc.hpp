@@ -3,6 +3,9 @@
 
 #include <stdint.h>
 
+#include <map>
+#include <memory>
+#include <span>
 #include <vector>
 
 #include "golay_gpu.h"
@@ -12,13 +15,29 @@ struct gp_ctx;
 
 namespace gp {
 
+// Split lists of one compression step c -> c' (uncompression.cpp:91-111,133-142),
+// per letter of the coarse alphabet: every r-tuple over the fine alphabet summing
+// to the letter (all permutations of each multiset, lexicographic order), and the
+// sorted dihedral-orbit minima of those tuples (used at position 0).
+struct SplitTable {
+    int r = 0;
+    std::map<int, int> letter_id;
+    std::vector<int> nsplit, nfirst;
+    std::vector<std::vector<int8_t>> split, first;  // [letter id][tuple * r + i]
+};
+std::shared_ptr<const SplitTable> make_splits(int compress, int new_compress);
+
 // One compressed sequence prepared for expansion (uncompression.cpp:85-142).
 struct SideHost {
     int L = 0, r = 0, N = 0, seqflag = 0;
-    std::vector<int> orig;                         // after the reference's rotation loop
-    std::vector<std::vector<std::vector<int>>> T;  // T[j][d] = r-tuple of position j
-    uint64_t leaves = 0;                           // product of |T[j]|
+    std::vector<int> orig;                 // after the reference's rotation loop
+    std::vector<int32_t> R;                // |T_j|: splits of position j
+    std::vector<const int8_t*> tp;         // T_j: R[j] r-tuples (into tab)
+    std::shared_ptr<const SplitTable> tab;
+    uint64_t leaves = 0;                   // product of R[j]
+    const int8_t* tuple(int j, int d) const { return tp[j] + (size_t)d * r; }
 };
+SideHost build_side(const std::shared_ptr<const SplitTable>& tab, const int8_t* orig, int L, int seqflag);
 SideHost build_side(const int8_t* orig, int L, int compress, int new_compress, int seqflag);
 
 struct UncSide;
@@ -49,9 +68,25 @@ struct UncEmitted {
     std::vector<int32_t> group;
 };
 
+// FP64 twiddles of length N (the context's PSD table), for unc_prepare.
+struct UncTrig {
+    int N = 0;
+    int64_t trig_off = 0;
+    std::vector<double> cs, sn;
+};
+UncTrig unc_trig(gp_ctx* ctx, int N);
+
+// Host tables of a batch of sides.  unc_prepare touches no device state and may
+// run on another host thread while the device works on the previous batch;
+// unc_launch uploads the tables and runs the filter (unc_filter = both).
+struct UncPrep;
+std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
+                                     bool fp32, const UncTrig& trig);
+UncBatch unc_launch(gp_ctx* ctx, UncPrep& prep, gp_stats* stats);
+
 // group[i]: pair index of input side i (a-side seqflag 0, b-side seqflag 1).
-UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::vector<int>& group, int order,
-                    int paf, gp_stats* stats);
+UncBatch unc_filter(gp_ctx* ctx, std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
+                    gp_stats* stats);
 UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* stats);
 // Survivors of one input side in DFS order with FP64 keys.
 CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side);

@@ -14,6 +14,8 @@
 // FP64 keys are computed for the emitted pairs only (their output order is the
 // text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
 // of each emitted pair is checked (counted in tie_band when it fails).
+#include <functional>
+#include <map>
 #include <thread>
 
 #include "unc_kernels.inc"  // opens namespace gp (closed at the end of this file)
@@ -286,21 +288,20 @@ double bf16_d(uint16_t h) {
     return f;
 }
 
+// Per-side tables (concatenated over the sides of a call).
 struct Tables {
     std::vector<UncSide> sides;
     std::vector<int32_t> radix;
-    std::vector<int8_t> tup, ent;
-    std::vector<float> c32, mid, w32;
-    std::vector<uint4> sufb, frags;
-    std::vector<double> c64;
+    std::vector<int8_t> tup;
     std::vector<uint64_t> pm1;
     std::vector<int32_t> pos32, pos64;  // per side and position: block offsets into Shared
     int64_t items = 0;
 };
 
-// Tables shared by every side of a call: the per-(position, letter) DFT
-// contribution blocks (FP32 channel layout and FP64 (Re, Im) pairs) and the
-// MFMA A fragments of each suffix start b.
+// Tables shared by every side of a call, content-addressed: the per-(position,
+// letter) DFT contribution blocks (FP32 channel layout and FP64 (Re, Im) pairs),
+// the MFMA A fragments and FP32 weights of each suffix start b, the mid tables of
+// each (a, b, letters at a..b-1) and the suffix operands of each (b, letters at b..L-1).
 struct Shared {
     std::map<std::pair<int, int>, int> block;  // (position j, letter at j) -> block id
     std::vector<int32_t> off32, off64;
@@ -309,14 +310,108 @@ struct Shared {
     std::map<int, int64_t> frag_off;  // suffix start b -> offset into frags
     std::map<int, double> rho;        // suffix start b -> max BF16 split residual
     std::vector<uint4> frags;
+    std::vector<float> mid, w32;      // mid tables; FP32 suffix weights (FP32 path)
+    std::vector<uint4> sufb;          // suffix B operands (split path)
+    std::vector<int8_t> ent;          // suffix entries (FP32 path)
+    std::map<int, int64_t> w32_off;   // suffix start b -> offset into w32
     int id(int j, int letter) const { return block.at({j, letter}); }
 };
+
+// Mid table of (a, b, letters): FP64 sums over the middle digits, rounded once
+// to FP32, [p_mid][mt*32] (last position fastest).
+void build_mid(const Shared& SH, const SideHost& S, int a, int b, std::vector<float>& out) {
+    const int K = S.N / 2, mt = (K + 15) / 16;
+    int64_t pm = 1;
+    for (int j = a; j < b; ++j) pm *= (int64_t)S.R[j];
+    out.assign((size_t)pm * mt * 32, 0.f);
+    std::vector<double> acc((size_t)2 * K);
+    std::vector<int> dg(S.L, 0);
+    for (int64_t m = 0; m < pm; ++m) {
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (int j = a; j < b; ++j) {
+            const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])] + (int64_t)dg[j] * K * 2];
+            for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
+        }
+        for (int k = 1; k <= K; ++k) {
+            const int t = (k - 1) / 16, row = (k - 1) % 16;
+            out[m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
+            out[m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
+        }
+        for (int j = b - 1; j >= a; --j) {
+            if (++dg[j] < S.R[j]) break;
+            dg[j] = 0;
+        }
+    }
+}
+
+// Suffix operands of (b, letters): BF16 B fragments [ntiles][nbs][64] (split
+// path) or int8 entries [p_suf][nent] (FP32 path); suffix combinations in
+// mixed-radix order, last position fastest.
+void build_suf(const SideHost& S, int b, bool fp32, std::vector<uint4>& sufb, std::vector<int8_t>& ent) {
+    const int L = S.L, r = S.r, nent = (L - b) * r;
+    int64_t ps = 1;
+    for (int j = b; j < L; ++j) ps *= (int64_t)S.R[j];
+    std::vector<int> sdg(L, 0);
+    std::vector<int8_t> vals((size_t)ps * nent);
+    for (int64_t s = 0; s < ps; ++s) {
+        for (int e = 0; e < nent; ++e) vals[s * nent + e] = S.tuple(b + e / r, sdg[b + e / r])[e % r];
+        for (int j = L - 1; j >= b; --j) {
+            if (++sdg[j] < S.R[j]) break;
+            sdg[j] = 0;
+        }
+    }
+    if (fp32) {
+        ent = std::move(vals);
+        return;
+    }
+    const int nbs = std::max(1, (nent + 15) / 16);
+    const int ntiles = (int)((ps + 31) / 32);
+    sufb.clear();
+    sufb.reserve((size_t)ntiles * nbs * 64);
+    for (int tile = 0; tile < ntiles; ++tile)
+        for (int blk = 0; blk < nbs; ++blk)
+            for (int lane = 0; lane < 64; ++lane) {
+                uint16_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                const int64_t s = (int64_t)tile * 32 + lane % 32;
+                if (s < ps)
+                    for (int i8 = 0; i8 < 8; ++i8) {
+                        const int e = 16 * blk + 8 * (lane / 32) + i8;
+                        if (e < nent) hv[i8] = bf16_rne_u((float)vals[s * nent + e]);
+                    }
+                uint4 q;
+                q.x = hv[0] | ((uint32_t)hv[1] << 16);
+                q.y = hv[2] | ((uint32_t)hv[3] << 16);
+                q.z = hv[4] | ((uint32_t)hv[5] << 16);
+                q.w = hv[6] | ((uint32_t)hv[7] << 16);
+                sufb.push_back(q);
+            }
+}
+
+// FP32 suffix DFT weights of suffix start b, [nent][mt*32] (FP32 path).
+void add_w32(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
+    if (SH.w32_off.count(b)) return;
+    const int K = N / 2, mt = (K + 15) / 16, nent = (L - b) * r;
+    SH.w32_off[b] = (int64_t)SH.w32.size();
+    for (int e = 0; e < nent; ++e) {
+        const int j = b + e / r, i = e % r, p = j + L * i;
+        for (int t = 0; t < mt; ++t)
+            for (int row = 0; row < 32; ++row) {
+                const int k = 16 * t + row % 16 + 1;
+                float v = 0.f;
+                if (k <= K) {
+                    const int mm = (int)(((int64_t)k * p) % N);
+                    v = (float)(row >= 16 ? sn[mm] : cs[mm]);
+                }
+                SH.w32.push_back(v);
+            }
+    }
+}
 
 void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& cs, const std::vector<double>& sn) {
     const auto key = std::make_pair(j, S.orig[j]);
     if (SH.block.count(key)) return;
     const int L = S.L, r = S.r, N = S.N, K = N / 2, mt = (K + 15) / 16;
-    const int64_t R = (int64_t)S.T[j].size();
+    const int64_t R = (int64_t)S.R[j];
     SH.block[key] = (int)SH.off32.size();
     SH.off32.push_back((int32_t)SH.c32.size());
     SH.off64.push_back((int32_t)SH.c64.size());
@@ -330,8 +425,8 @@ void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& 
             for (int i = 0; i < r; ++i) {
                 const int p = j + L * i;
                 const int mm = (int)(((int64_t)k * p) % N);
-                re += S.T[j][d][i] * cs[mm];
-                im += S.T[j][d][i] * sn[mm];
+                re += S.tuple(j, d)[i] * cs[mm];
+                im += S.tuple(j, d)[i] * sn[mm];
             }
             c64[(d * K + (k - 1)) * 2] = re;
             c64[(d * K + (k - 1)) * 2 + 1] = im;
@@ -384,14 +479,14 @@ void side_split(const SideHost& S, int& a, int& b) {
     const int L = S.L;
     b = L;
     int64_t ps = 1;
-    while (b > 0 && ps < 64) { --b; ps *= (int64_t)S.T[b].size(); }
+    while (b > 0 && ps < 64) { --b; ps *= (int64_t)S.R[b]; }
     a = b;
     int64_t pm = 1;
-    while (a > 0 && pm < 16 && pm * (int64_t)S.T[a - 1].size() <= 4096) { --a; pm *= (int64_t)S.T[a].size(); }
+    while (a > 0 && pm < 16 && pm * (int64_t)S.R[a - 1] <= 4096) { --a; pm *= (int64_t)S.R[a]; }
 }
 
-void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int order, int paf,
-              const std::vector<double>& cs, const std::vector<double>& sn, int group, bool fp32) {
+void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int order, int paf, int group, bool fp32,
+              int64_t mid_off, int64_t suf_off) {
     UncSide D{};
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
@@ -399,7 +494,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
     const int L = S.L, r = S.r, N = S.N, K = D.K;
     const int mt = (K + 15) / 16;
     std::vector<int64_t> R(L);
-    for (int j = 0; j < L; ++j) R[j] = (int64_t)S.T[j].size();
+    for (int j = 0; j < L; ++j) R[j] = (int64_t)S.R[j];
     // split: suffix [b, L) with p_suf >= 64, mid [a, b) with 16 <= p_mid <= 4096
     int a, b;
     side_split(S, a, b);
@@ -421,7 +516,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
     for (int j = 0; j < L; ++j)
         for (int64_t d = 0; d < R[j]; ++d)
             for (int i = 0; i < r; ++i) {
-                const int v = S.T[j][d][i];
+                const int v = S.tuple(j, d)[i];
                 T.tup[D.tup_off + ((int64_t)j * rmax + d) * r + i] = (int8_t)v;
                 if (v != 1 && v != -1) pm1_ok = false;
             }
@@ -433,7 +528,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
             for (int64_t d = 0; d < R[j]; ++d) {
                 uint64_t msk = 0;
                 for (int i = 0; i < r; ++i)
-                    if (S.T[j][d][i] == 1) msk |= 1ull << (j + L * i);
+                    if (S.tuple(j, d)[i] == 1) msk |= 1ull << (j + L * i);
                 T.pm1[D.pm1_off + (int64_t)j * rmax + d] = msk;
             }
     }
@@ -445,77 +540,10 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
         T.pos32.push_back(SH.off32[id]);
         T.pos64.push_back(SH.off64[id]);
     }
-    // mid table: FP64 sums over the middle digits, rounded once to FP32
-    D.mid_off = (int64_t)T.mid.size();
-    T.mid.resize(T.mid.size() + (size_t)pm * mt * 32, 0.f);
-    {
-        std::vector<double> acc((size_t)2 * K);
-        std::vector<int> dg(L, 0);
-        for (int64_t m = 0; m < pm; ++m) {
-            std::fill(acc.begin(), acc.end(), 0.0);
-            for (int j = a; j < b; ++j) {
-                const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])] + (int64_t)dg[j] * K * 2];
-                for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
-            }
-            for (int k = 1; k <= K; ++k) {
-                const int t = (k - 1) / 16, row = (k - 1) % 16;
-                T.mid[D.mid_off + m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
-                T.mid[D.mid_off + m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
-            }
-            for (int j = b - 1; j >= a; --j) {  // next mid combination (last position fastest)
-                if (++dg[j] < R[j]) break;
-                dg[j] = 0;
-            }
-        }
-    }
-    // suffix combinations: BF16 B operands (split path) or int8 entries (FP32 path)
-    std::vector<int> sdg(L, 0);
-    std::vector<std::vector<int>> sufvals((size_t)ps, std::vector<int>(nent, 0));
-    for (int64_t s = 0; s < ps; ++s) {
-        for (int e = 0; e < nent; ++e) sufvals[s][e] = S.T[b + e / r][sdg[b + e / r]][e % r];
-        for (int j = L - 1; j >= b; --j) {
-            if (++sdg[j] < R[j]) break;
-            sdg[j] = 0;
-        }
-    }
-    D.sufb_off = (int64_t)T.sufb.size();
-    D.ent_off = (int64_t)T.ent.size();
-    D.w32_off = (int64_t)T.w32.size();
-    if (!fp32) {
-        for (int tile = 0; tile < D.ntiles; ++tile)
-            for (int blk = 0; blk < D.nbs; ++blk)
-                for (int lane = 0; lane < 64; ++lane) {
-                    uint16_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                    const int64_t s = (int64_t)tile * 32 + lane % 32;
-                    if (s < ps)
-                        for (int i8 = 0; i8 < 8; ++i8) {
-                            const int e = 16 * blk + 8 * (lane / 32) + i8;
-                            if (e < nent) hv[i8] = bf16_rne_u((float)sufvals[s][e]);
-                        }
-                    uint4 q;
-                    q.x = hv[0] | ((uint32_t)hv[1] << 16);
-                    q.y = hv[2] | ((uint32_t)hv[3] << 16);
-                    q.z = hv[4] | ((uint32_t)hv[5] << 16);
-                    q.w = hv[6] | ((uint32_t)hv[7] << 16);
-                    T.sufb.push_back(q);
-                }
-    } else {
-        for (int64_t s = 0; s < ps; ++s)
-            for (int e = 0; e < nent; ++e) T.ent.push_back((int8_t)sufvals[s][e]);
-        for (int e = 0; e < nent; ++e) {
-            const int j = b + e / r, i = e % r, p = j + L * i;
-            for (int t = 0; t < mt; ++t)
-                for (int row = 0; row < 32; ++row) {
-                    const int k = 16 * t + row % 16 + 1;
-                    float v = 0.f;
-                    if (k <= K) {
-                        const int mm = (int)(((int64_t)k * p) % N);
-                        v = (float)(row >= 16 ? sn[mm] : cs[mm]);
-                    }
-                    T.w32.push_back(v);
-                }
-        }
-    }
+    D.mid_off = mid_off;
+    D.sufb_off = fp32 ? 0 : suf_off;
+    D.ent_off = fp32 ? suf_off : 0;
+    D.w32_off = fp32 ? SH.w32_off.at(b) : 0;
     // A fragments (suffix DFT weights, BF16 hi and lo), shared per suffix start
     D.frag_off = fp32 ? 0 : SH.frag_off.at(b);
     const double rho = fp32 ? 0.0 : SH.rho.at(b);
@@ -525,7 +553,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
         double mxabs = 0;
         for (int64_t d = 0; d < R[j]; ++d) {
             double ab = 0;
-            for (int i = 0; i < r; ++i) ab += std::abs(S.T[j][d][i]);
+            for (int i = 0; i < r; ++i) ab += std::abs(S.tuple(j, d)[i]);
             mxabs = std::max(mxabs, ab);
         }
         (j < b ? s1_pre : s1_suf) += mxabs;
@@ -559,88 +587,163 @@ V* upload(gp_ctx* ctx, const char* name, const std::vector<V>& v) {
 // ---------------------------------------------------------------------------
 // host: the three stages
 
-UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::vector<int>& group, int order,
-                    int paf, gp_stats* stats) {
-    UncBatch B;
-    if (sides.empty()) return B;
-    hipStream_t st = ctx->stream;
-    const int N = sides[0].N;
+// Host tables of one batch (built without the device, so that the next batch's
+// tables can be prepared while the current batch runs).
+struct UncPrep {
+    int N = 0, bound = 0, rmax = 1;
+    bool fp32 = false;
+    int64_t trig_off = 0;
+    Tables T;
+    Shared SH;
+    std::vector<int> side_of;
+    uint64_t total_leaves = 0;
+};
+
+UncTrig unc_trig(gp_ctx* ctx, int N) {
     if (N > kMaxN || N < 2) throw ArgError("uncompress: expanded length outside [2, 64]");
+    const int table = ctx->tables.get(N);
+    ctx->tables.upload(ctx->stream);
+    const PsdTableDesc& td = ctx->tables.desc[table];
+    UncTrig t;
+    t.N = N;
+    t.trig_off = td.trig_off;
+    t.cs.assign(ctx->tables.trig64.begin() + td.trig_off, ctx->tables.trig64.begin() + td.trig_off + N);
+    t.sn.assign(ctx->tables.trig64.begin() + td.trig_off + N, ctx->tables.trig64.begin() + td.trig_off + 2 * N);
+    return t;
+}
+
+std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
+                                     bool fp32, const UncTrig& trig) {
+    auto prep = std::make_shared<UncPrep>();
+    UncPrep& U = *prep;
+    if (sides.empty()) return prep;
+    const int N = sides[0].N;
+    if (N != trig.N) throw ArgError("uncompress: twiddle table length mismatch");
     if ((int64_t)sides.size() >= (1ll << 23)) throw ArgError("uncompress: too many sequences in one batch");
     const int bound = 2 * order - paf;
     if (bound < 0 || bound > 255) throw ArgError("uncompress: keys need 0 <= 2*order - paf <= 255");
-    const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
     int rmax = 1;
     for (const auto& s : sides)
-        for (const auto& t : s.T) rmax = std::max<int>(rmax, (int)t.size());
-    const int table = ctx->tables.get(N);
-    ctx->tables.upload(st);
-    const PsdTableDesc& td = ctx->tables.desc[table];
-    std::vector<double> cs(ctx->tables.trig64.begin() + td.trig_off, ctx->tables.trig64.begin() + td.trig_off + N);
-    std::vector<double> sn(ctx->tables.trig64.begin() + td.trig_off + N,
-                           ctx->tables.trig64.begin() + td.trig_off + 2 * N);
-    Tables T;
-    Shared SH;
-    uint64_t total_leaves = 0;
+        for (int R : s.R) rmax = std::max(rmax, R);
+    U.N = N;
+    U.bound = bound;
+    U.rmax = rmax;
+    U.fp32 = fp32;
+    U.trig_off = trig.trig_off;
+    const std::vector<double>& cs = trig.cs;
+    const std::vector<double>& sn = trig.sn;
+    Tables& T = U.T;
+    Shared& SH = U.SH;
+    uint64_t& total_leaves = U.total_leaves;
+    std::vector<int>& side_of = U.side_of;
     {
         Phase ph("unc_filter: side tables");
         std::vector<size_t> live;
         for (size_t i = 0; i < sides.size(); ++i)
             if (sides[i].leaves != 0) live.push_back(i);
-        // shared per-(position, letter) blocks and per-suffix-start fragments
-        for (size_t i : live) {
-            for (int j = 0; j < sides[i].L; ++j) add_block(SH, sides[i], j, cs, sn);
-            if (!fp32) {
-                int a, b;
-                side_split(sides[i], a, b);
-                add_frags(SH, b, sides[i].L, sides[i].r, N, cs, sn);
-            }
+        // shared per-(position, letter) blocks and per-suffix-start fragments / weights
+        std::vector<int> sa(live.size()), sb(live.size());
+        Phase ph1("tables: blocks + keys");
+        for (size_t q = 0; q < live.size(); ++q) {
+            const SideHost& S = sides[live[q]];
+            for (int j = 0; j < S.L; ++j) add_block(SH, S, j, cs, sn);
+            side_split(S, sa[q], sb[q]);
+            if (!fp32) add_frags(SH, sb[q], S.L, S.r, N, cs, sn);
+            else add_w32(SH, sb[q], S.L, S.r, N, cs, sn);
         }
+        // content-addressed mid and suffix tables: one job per distinct key
+        std::map<std::vector<int>, int> mid_job, suf_job;
+        std::vector<size_t> mid_rep, suf_rep;  // representative side (index into live) per job
+        std::vector<int> side_mid(live.size()), side_suf(live.size());
+        for (size_t q = 0; q < live.size(); ++q) {
+            const SideHost& S = sides[live[q]];
+            std::vector<int> km{sa[q], sb[q]}, ks{sb[q]};
+            km.insert(km.end(), S.orig.begin() + sa[q], S.orig.begin() + sb[q]);
+            ks.insert(ks.end(), S.orig.begin() + sb[q], S.orig.end());
+            auto im = mid_job.emplace(km, (int)mid_rep.size());
+            if (im.second) mid_rep.push_back(q);
+            side_mid[q] = im.first->second;
+            auto is = suf_job.emplace(ks, (int)suf_rep.size());
+            if (is.second) suf_rep.push_back(q);
+            side_suf[q] = is.first->second;
+        }
+        const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+        auto parallel = [nt](size_t count, const std::function<void(size_t)>& f) {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t] {
+                    for (size_t q = t; q < count; q += nt) f(q);
+                });
+            for (auto& x : th) x.join();
+        };
+        ph1.next("tables: mid/suffix jobs");
+        std::vector<std::vector<float>> mids(mid_rep.size());
+        std::vector<std::vector<uint4>> sufbs(suf_rep.size());
+        std::vector<std::vector<int8_t>> ents(suf_rep.size());
+        parallel(mid_rep.size() + suf_rep.size(), [&](size_t i) {
+            if (i < mid_rep.size()) {
+                const size_t q = mid_rep[i];
+                build_mid(SH, sides[live[q]], sa[q], sb[q], mids[i]);
+            } else {
+                const size_t q = suf_rep[i - mid_rep.size()];
+                build_suf(sides[live[q]], sb[q], fp32, sufbs[i - mid_rep.size()], ents[i - mid_rep.size()]);
+            }
+        });
+        std::vector<int64_t> mid_off(mids.size()), suf_off(suf_rep.size());
+        for (size_t i = 0; i < mids.size(); ++i) {
+            mid_off[i] = (int64_t)SH.mid.size();
+            SH.mid.insert(SH.mid.end(), mids[i].begin(), mids[i].end());
+        }
+        for (size_t i = 0; i < suf_rep.size(); ++i) {
+            suf_off[i] = fp32 ? (int64_t)SH.ent.size() : (int64_t)SH.sufb.size();
+            SH.sufb.insert(SH.sufb.end(), sufbs[i].begin(), sufbs[i].end());
+            SH.ent.insert(SH.ent.end(), ents[i].begin(), ents[i].end());
+        }
+        ph1.next("tables: sides + merge");
         // per-side tables on host threads, then concatenated with rebased offsets
         std::vector<Tables> part(live.size());
-        const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&, t] {
-                for (size_t q = t; q < live.size(); q += nt)
-                    add_side(part[q], SH, sides[live[q]], rmax, order, paf, cs, sn, group[live[q]], fp32);
-            });
-        for (auto& x : th) x.join();
+        parallel(live.size(), [&](size_t q) {
+            add_side(part[q], SH, sides[live[q]], rmax, order, paf, group[live[q]], fp32, mid_off[side_mid[q]],
+                     suf_off[side_suf[q]]);
+        });
         for (size_t q = 0; q < live.size(); ++q) {
             UncSide D = part[q].sides[0];
             D.item_base = T.items;
             D.radix_off += (int64_t)T.radix.size();
             D.tup_off += (int64_t)T.tup.size();
             D.c32_off += (int64_t)T.pos32.size();
-            D.mid_off += (int64_t)T.mid.size();
-            D.sufb_off += (int64_t)T.sufb.size();
             D.c64_off += (int64_t)T.pos64.size();
-            D.ent_off += (int64_t)T.ent.size();
-            D.w32_off += (int64_t)T.w32.size();
             if (D.pm1_off >= 0) D.pm1_off += (int64_t)T.pm1.size();
             T.items += part[q].items;
             auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
             app(T.radix, part[q].radix);
             app(T.tup, part[q].tup);
             app(T.pos32, part[q].pos32);
-            app(T.mid, part[q].mid);
-            app(T.sufb, part[q].sufb);
             app(T.pos64, part[q].pos64);
-            app(T.ent, part[q].ent);
-            app(T.w32, part[q].w32);
             app(T.pm1, part[q].pm1);
             T.sides.push_back(D);
-            B.side_of.push_back((int)live[q]);
+            side_of.push_back((int)live[q]);
             total_leaves += sides[live[q]].leaves;
-            Tables().sides.swap(part[q].sides);
             part[q] = Tables();
         }
     }
-    B.N = N;
-    B.K = N / 2;
-    B.trig_off = td.trig_off;
-    B.bound = bound;
+    return prep;
+}
+
+UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
+    UncBatch B;
+    hipStream_t st = ctx->stream;
+    const bool fp32 = U.fp32;
+    const Tables& T = U.T;
+    const Shared& SH = U.SH;
+    const uint64_t total_leaves = U.total_leaves;
+    B.N = U.N;
+    B.K = U.N / 2;
+    B.trig_off = U.trig_off;
+    B.bound = U.bound;
+    B.side_of = U.side_of;
     if (T.sides.empty()) return B;
+    Phase ph2("unc_filter: upload");
     B.d_sides = upload(ctx, "u_sides", T.sides);
     B.d_radix = upload(ctx, "u_radix", T.radix);
     B.d_tup = upload(ctx, "u_tup", T.tup);
@@ -649,12 +752,12 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
     B.d_pos64 = upload(ctx, "u_pos64", T.pos64);
     auto* d_pos32 = upload(ctx, "u_pos32", T.pos32);
     auto* d_c32 = upload(ctx, "u_c32", SH.c32);
-    auto* d_mid = upload(ctx, "u_mid", T.mid);
-    auto* d_sufb = upload(ctx, "u_sufb", T.sufb);
+    auto* d_mid = upload(ctx, "u_mid", SH.mid);
+    auto* d_sufb = upload(ctx, "u_sufb", SH.sufb);
     auto* d_frags = upload(ctx, "u_frags", SH.frags);
-    auto* d_ent = upload(ctx, "u_ent", T.ent);
-    auto* d_w32 = upload(ctx, "u_w32", T.w32);
-    std::vector<int64_t> toff{td.trig_off};
+    auto* d_ent = upload(ctx, "u_ent", SH.ent);
+    auto* d_w32 = upload(ctx, "u_w32", SH.w32);
+    std::vector<int64_t> toff{U.trig_off};
     auto* d_toff = upload(ctx, "u_toff", toff);
     B.nsides_table = (int)T.sides.size();
     UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
@@ -662,6 +765,7 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
     int64_t bcap = (int64_t)std::min<uint64_t>(total_leaves / 64 + (1 << 16), (uint64_t)1 << 28);
     UncCtl h{};
     float ms_filter = 0, ms_band = 0;
+    ph2.next("unc_filter: kernels");
     for (int attempt = 0; attempt < 3; ++attempt) {
         uint64_t* d_surv = ctx->dev<uint64_t>("u_surv", (size_t)scap);
         uint64_t* d_band = ctx->dev<uint64_t>("u_band", (size_t)bcap);
@@ -707,6 +811,14 @@ UncBatch unc_filter(gp_ctx* ctx, const std::vector<SideHost>& sides, const std::
         stats->kernel_ms += ms_filter + ms_band;
     }
     return B;
+}
+
+UncBatch unc_filter(gp_ctx* ctx, std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
+                    gp_stats* stats) {
+    if (sides.empty()) return UncBatch{};
+    const UncTrig trig = unc_trig(ctx, sides[0].N);
+    auto prep = unc_prepare(sides, group, order, paf, ctx->psd_mode == GP_PSD_FP32, trig);
+    return unc_launch(ctx, *prep, stats);
 }
 
 UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* stats) {
