@@ -9,6 +9,7 @@ Each check is independent of the restatement it tests:
   check_if_pair                   numpy PAF
   gen                             brute force over all words (numpy FFT, orbit min)
   match                           nested-loop join on key text
+  sort                            coreutils `sort | uniq` under LC_ALL=C on the gen files
   pair group (filter)             closure == parametric family of signed affine maps
   cache_filter                    brute-force orbits, lexicographically first input member
   uncompress                      compress(expanded word) == rotated original; leaf count
@@ -17,7 +18,8 @@ Each check is independent of the restatement it tests:
                                   compression route (v=16, 20, 26)
 Parity unpinned (code-following only): the uncompression rotation quirk and
 first-position representatives (uncompression.cpp:113-142) beyond the invariants
-above, and GNU sort's order (pinned as C-locale byte order).
+above; FFTW's last-ulp PSD values (a direct FP64 DFT stands in; the tie-band
+counter reports any value within 1e-9 of a rounding or filter threshold).
 """
 import itertools
 import math
@@ -333,3 +335,17 @@ def test_v34_filter_known_answer(oracle):
         got = open(os.path.join(wd, "34-unique")).read()
     assert got.count("\n") == 373
     assert got == open(os.path.join(GOLDEN, "v34", "34-unique")).read()
+
+
+@pytest.mark.parametrize("args", [[26, "-c", 2], [45, "-c", 3, "--paf", -2]])
+def test_sorted_files_equal_gnu_sort_uniq(oracle, args, tmp_path):
+    # sort.cpp:17 runs `sort ... | uniq`; the oracle's sort must give GNU sort's
+    # byte order under LC_ALL=C (coreutils sort and uniq are run here as the check)
+    d = str(tmp_path)
+    assert _cli(list(args) + ["gen", "sort"], d).returncode == 0
+    wd = os.path.join(d, "order-%d" % args[0])
+    env = dict(os.environ, LC_ALL="C")
+    for ab in "ab":
+        gen = os.path.join(wd, "%d-filtered-%s#0" % (args[0], ab))
+        ref = subprocess.run("sort '%s' | uniq" % gen, shell=True, env=env, capture_output=True, check=True).stdout
+        assert open(os.path.join(wd, "%d-filtered_sorted-%s" % (args[0], ab)), "rb").read() == ref
