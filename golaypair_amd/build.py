@@ -23,6 +23,8 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
     "-Wall", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs (no accvgpr moves around the PSD reduction)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC]

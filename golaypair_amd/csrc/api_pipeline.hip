@@ -8,6 +8,7 @@
 #include "gp_internal.hpp"
 #include "gp_sort.hpp"
 #include "psd_core.hpp"
+#include "linesort.hpp"
 #include "textfmt.hpp"
 
 using namespace gp;
@@ -109,16 +110,13 @@ int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
         const int n = s.n, K = s.nkey;
         if (K != n / 2) throw ArgError("gp_sort_candidates: nkey must be n/2");
         std::string text;
-        std::vector<size_t> off(m + 1);
-        for (int64_t i = 0; i < m; ++i) {
-            off[i] = text.size();
-            append_cand_line(text, &s.words[i * n], &s.keys[i * K], n);
-        }
-        off[m] = text.size();
+        std::vector<size_t> off;
+        build_lines(m, [&](std::string& t, int64_t i) { append_cand_line(t, &s.words[i * n], &s.keys[i * K], n); },
+                    text, off);
         std::vector<int64_t> idx(m);
         std::iota(idx.begin(), idx.end(), 0);
         auto line = [&](int64_t i) { return std::string_view(text.data() + off[i], off[i + 1] - off[i]); };
-        std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return line(x) < line(y); });
+        sort_lines(LineSet{&text, &off, 3}, idx);  // every line starts with "PSD"
         int64_t w = 0;
         for (int64_t r = 0; r < m; ++r) {
             if (w > 0 && line(idx[r]) == line(idx[w - 1])) continue;  // uniq

@@ -10,6 +10,7 @@
 #include "gp_host.hpp"
 #include "gp_internal.hpp"
 #include "gp_unc.hpp"
+#include "linesort.hpp"
 #include "textfmt.hpp"
 
 using namespace gp;
@@ -158,11 +159,17 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             }
             // per input pair: emission order (A line, B line) in text order
             const int K = N / 2;
-            std::vector<std::string> la(E.count), lb(E.count);
-            for (int64_t i = 0; i < E.count; ++i) {
-                append_cand_line(la[i], &E.words[i * N], &E.keys[i * K], N);
-                append_cand_line(lb[i], &E.words[(E.count + i) * N], &E.keys[(E.count + i) * K], N);
-            }
+            std::string ta, tb;
+            std::vector<size_t> oa, ob;
+            build_lines(E.count, [&](std::string& t, int64_t i) {
+                append_cand_line(t, &E.words[i * N], &E.keys[i * K], N);
+            }, ta, oa);
+            build_lines(E.count, [&](std::string& t, int64_t i) {
+                append_cand_line(t, &E.words[(E.count + i) * N], &E.keys[(E.count + i) * K], N);
+            }, tb, ob);
+            const LineSet LA{&ta, &oa, 3}, LB{&tb, &ob, 3};
+            std::vector<uint64_t> ha, la;
+            line_keys(LA, E.count, ha, la);
             std::vector<int64_t> ord;
             for (int64_t i = 0; i < E.count; ++i) {
                 // key texts must agree (they do unless an FP64 PSD sits on a rounding tie)
@@ -174,7 +181,10 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             }
             std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
                 if (E.group[x] != E.group[y]) return E.group[x] < E.group[y];
-                return la[x] != la[y] ? la[x] < la[y] : lb[x] < lb[y];
+                if (ha[x] != ha[y]) return ha[x] < ha[y];
+                if (la[x] != la[y]) return la[x] < la[y];
+                const std::string_view ax = LA.line(x), ay = LA.line(y);
+                return ax != ay ? ax < ay : LB.line(x) < LB.line(y);
             });
             for (int64_t i : ord) {
                 result.a.insert(result.a.end(), &E.words[i * N], &E.words[i * N] + N);

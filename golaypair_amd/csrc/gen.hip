@@ -258,6 +258,7 @@ std::vector<int> units_of(int n) {
 int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int proc_num, GenOutput& out,
                    gp_stats* stats) {
     hipStream_t st = ctx->stream;
+    Phase ph("gen: setup");
     const int n = order / compress;
     if (n < 2 || n > 32) throw ArgError("gen: word length " + std::to_string(n) + " outside [2, 32]");
     const std::vector<int> alpha = getalphabet(compress);
@@ -339,6 +340,7 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     if (!bits.empty())
         GP_HIP(hipMemcpyAsync(d_bits.ensure(bits.size() * 4), bits.data(), bits.size() * 4, hipMemcpyHostToDevice, st));
     GenCtl h{};
+    ph.next("gen: enumerate");
     for (int attempt = 0; attempt < 2; ++attempt) {
         P.capacity = capacity;
         u128* d_w = static_cast<u128*>(ctx->buf("g_w").ensure((size_t)capacity * sizeof(u128)));
@@ -364,6 +366,7 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     const int64_t m = (int64_t)h.survivors;
     if (stats) { stats->words += h.words; stats->canonical += (uint64_t)m; }
     // PSD filter on the canonical survivors
+    ph.next("gen: psd + keys");
     const int table = ctx->tables.get(n);
     const PsdTableDesc& d = ctx->tables.desc[table];
     const int stride = 16 * d.nb;
@@ -413,13 +416,21 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         if (stats) stats->tie_band += ties;
     }
     // collect A and B in the reference's file order (lexicographic order of the words)
+    ph.next("gen: order");
     const int K = n / 2;
     std::vector<int64_t> ia, ib;
     for (int64_t i = 0; i < acc_n; ++i) (side_h[i] ? ib : ia).push_back(i);
     if (stats) stats->psd_pass += ia.size() + ib.size();
+    // packed big-endian keys (byte v + 128 per letter): unsigned order = signed lexicographic order
+    const int W = (n + 7) / 8;
+    std::vector<uint64_t> pk((size_t)acc_n * W, 0);
+    for (int64_t i = 0; i < acc_n; ++i)
+        for (int j = 0; j < n; ++j)
+            pk[i * W + j / 8] |= (uint64_t)(uint8_t)(words_h[i * n + j] + 128) << (56 - 8 * (j % 8));
     auto lexless = [&](int64_t x, int64_t y) {
-        return std::lexicographical_compare(&words_h[x * n], &words_h[x * n] + n, &words_h[y * n],
-                                            &words_h[y * n] + n);
+        for (int w = 0; w < W; ++w)
+            if (pk[x * W + w] != pk[y * W + w]) return pk[x * W + w] < pk[y * W + w];
+        return false;
     };
     std::sort(ia.begin(), ia.end(), lexless);
     std::sort(ib.begin(), ib.end(), lexless);

@@ -1,0 +1,105 @@
+// Bytewise (C locale) sort of text lines held in one buffer, as `sort` does
+// under LC_ALL=C (src/core/sort/sort.cpp:17): each line gets a 16-byte
+// big-endian prefix key (zero padded, so a proper prefix sorts first, every
+// text byte being >= 0x20), and ties on the key fall back to comparing the rest
+// of the lines.  Chunks are sorted on host threads and merged pairwise.
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+#include <string_view>
+#include <thread>
+#include <vector>
+
+namespace gp {
+
+struct LineSet {
+    const std::string* text = nullptr;
+    const std::vector<size_t>* off = nullptr;  // line i = text[off[i], off[i+1])
+    size_t skip = 0;                           // bytes every line shares (not keyed)
+    std::string_view line(int64_t i) const {
+        return std::string_view(text->data() + (*off)[i], (*off)[i + 1] - (*off)[i]);
+    }
+};
+
+// Builds m lines into one buffer on host threads: append(str, i) appends line i.
+template <class F>
+void build_lines(int64_t m, F append, std::string& text, std::vector<size_t>& off, int threads = 16) {
+    const int nt = std::max(1, std::min<int>(threads, (int)(m / 2048)));
+    std::vector<std::string> part(nt);
+    std::vector<std::vector<size_t>> poff(nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            const int64_t a = m * t / nt, b = m * (t + 1) / nt;
+            for (int64_t i = a; i < b; ++i) {
+                poff[t].push_back(part[t].size());
+                append(part[t], i);
+            }
+        });
+    for (auto& x : th) x.join();
+    size_t total = 0;
+    for (auto& p : part) total += p.size();
+    text.clear();
+    text.reserve(total);
+    off.clear();
+    off.reserve(m + 1);
+    for (int t = 0; t < nt; ++t) {
+        const size_t base = text.size();
+        for (size_t o : poff[t]) off.push_back(base + o);
+        text += part[t];
+    }
+    off.push_back(text.size());
+}
+
+inline void line_keys(const LineSet& L, int64_t m, std::vector<uint64_t>& hi, std::vector<uint64_t>& lo) {
+    hi.assign(m, 0);
+    lo.assign(m, 0);
+    for (int64_t i = 0; i < m; ++i) {
+        const std::string_view s = L.line(i);
+        uint64_t h = 0, l = 0;
+        for (size_t b = 0; b < 16; ++b) {
+            const size_t p = L.skip + b;
+            const uint64_t c = p < s.size() ? (uint8_t)s[p] : 0;
+            if (b < 8) h |= c << (56 - 8 * b);
+            else l |= c << (56 - 8 * (b - 8));
+        }
+        hi[i] = h;
+        lo[i] = l;
+    }
+}
+
+// idx: a permutation of [0, m); sorted into line order.
+inline void sort_lines(const LineSet& L, std::vector<int64_t>& idx, int threads = 16) {
+    const int64_t m = (int64_t)idx.size();
+    std::vector<uint64_t> hi, lo;
+    line_keys(L, m, hi, lo);
+    auto less = [&](int64_t x, int64_t y) {
+        if (hi[x] != hi[y]) return hi[x] < hi[y];
+        if (lo[x] != lo[y]) return lo[x] < lo[y];
+        const std::string_view a = L.line(x), b = L.line(y);
+        const size_t k = std::min(L.skip + 16, std::min(a.size(), b.size()));
+        return a.substr(k) < b.substr(k);
+    };
+    int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
+    std::vector<int64_t> cut(nt + 1);
+    for (int t = 0; t <= nt; ++t) cut[t] = m * t / nt;
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
+        for (auto& x : th) x.join();
+    }
+    for (int w = 1; w < nt; w *= 2) {
+        std::vector<std::thread> th;
+        for (int t = 0; t + w < nt; t += 2 * w) {
+            const int64_t a = cut[t], b = cut[t + w], e = cut[std::min(t + 2 * w, nt)];
+            th.emplace_back([&, a, b, e] { std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less); });
+        }
+        for (auto& x : th) x.join();
+    }
+}
+
+}  // namespace gp

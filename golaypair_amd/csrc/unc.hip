@@ -15,6 +15,8 @@
 // text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
 // of each emitted pair is checked (counted in tie_band when it fails).
 #include <functional>
+#include <numeric>
+#include <tuple>
 #include <map>
 #include <thread>
 
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restri
         while (sid + 1 < nsides && it >= sides[sid + 1].item_base) ++sid;
         const UncSide& S = sides[sid];
         const int64_t o = it - S.item_base;
-        const int mt = (S.K + 15) / 16;
+        const int mt = (S.nch + 15) / 16;
         const int nch = mt * 32;
         const int nent = (S.L - S.b) * S.r;
         // X_outer, one channel per lane
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restri
                 float mx = 0.f;
                 if (valid) {
                     const int8_t* e = ent + S.ent_off + s * nent;
-                    for (int k = 0; k < S.K; ++k) {
+                    for (int k = 0; k < S.nch; ++k) {
                         const int t = k >> 4, row = k & 15;
                         float re = xpre[wv][t * 32 + row], im = xpre[wv][t * 32 + 16 + row];
                         for (int q = 0; q < nent; ++q) {
@@ -314,13 +316,16 @@ struct Shared {
     std::vector<uint4> sufb;          // suffix B operands (split path)
     std::vector<int8_t> ent;          // suffix entries (FP32 path)
     std::map<int, int64_t> w32_off;   // suffix start b -> offset into w32
+    // FP32 filter channels: slot q (row q % 16 of M-tile q / 16) holds k = chan[q]
+    std::vector<int> chan;
+    int mt() const { return ((int)chan.size() + 15) / 16; }
     int id(int j, int letter) const { return block.at({j, letter}); }
 };
 
 // Mid table of (a, b, letters): FP64 sums over the middle digits, rounded once
 // to FP32, [p_mid][mt*32] (last position fastest).
 void build_mid(const Shared& SH, const SideHost& S, int a, int b, std::vector<float>& out) {
-    const int K = S.N / 2, mt = (K + 15) / 16;
+    const int K = S.N / 2, mt = SH.mt();
     int64_t pm = 1;
     for (int j = a; j < b; ++j) pm *= (int64_t)S.R[j];
     out.assign((size_t)pm * mt * 32, 0.f);
@@ -332,8 +337,8 @@ void build_mid(const Shared& SH, const SideHost& S, int a, int b, std::vector<fl
             const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])] + (int64_t)dg[j] * K * 2];
             for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
         }
-        for (int k = 1; k <= K; ++k) {
-            const int t = (k - 1) / 16, row = (k - 1) % 16;
+        for (size_t q = 0; q < SH.chan.size(); ++q) {
+            const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
             out[m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
             out[m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
         }
@@ -390,16 +395,16 @@ void build_suf(const SideHost& S, int b, bool fp32, std::vector<uint4>& sufb, st
 // FP32 suffix DFT weights of suffix start b, [nent][mt*32] (FP32 path).
 void add_w32(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
     if (SH.w32_off.count(b)) return;
-    const int K = N / 2, mt = (K + 15) / 16, nent = (L - b) * r;
+    const int mt = SH.mt(), nent = (L - b) * r;
     SH.w32_off[b] = (int64_t)SH.w32.size();
     for (int e = 0; e < nent; ++e) {
         const int j = b + e / r, i = e % r, p = j + L * i;
         for (int t = 0; t < mt; ++t)
             for (int row = 0; row < 32; ++row) {
-                const int k = 16 * t + row % 16 + 1;
+                const size_t q = 16 * t + row % 16;
                 float v = 0.f;
-                if (k <= K) {
-                    const int mm = (int)(((int64_t)k * p) % N);
+                if (q < SH.chan.size()) {
+                    const int mm = (int)(((int64_t)SH.chan[q] * p) % N);
                     v = (float)(row >= 16 ? sn[mm] : cs[mm]);
                 }
                 SH.w32.push_back(v);
@@ -410,7 +415,7 @@ void add_w32(Shared& SH, int b, int L, int r, int N, const std::vector<double>& 
 void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& cs, const std::vector<double>& sn) {
     const auto key = std::make_pair(j, S.orig[j]);
     if (SH.block.count(key)) return;
-    const int L = S.L, r = S.r, N = S.N, K = N / 2, mt = (K + 15) / 16;
+    const int L = S.L, r = S.r, N = S.N, K = N / 2, mt = SH.mt();
     const int64_t R = (int64_t)S.R[j];
     SH.block[key] = (int)SH.off32.size();
     SH.off32.push_back((int32_t)SH.c32.size());
@@ -430,15 +435,18 @@ void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& 
             }
             c64[(d * K + (k - 1)) * 2] = re;
             c64[(d * K + (k - 1)) * 2 + 1] = im;
-            const int t = (k - 1) / 16, row = (k - 1) % 16;
-            c32[d * mt * 32 + t * 32 + row] = (float)re;
-            c32[d * mt * 32 + t * 32 + 16 + row] = (float)im;
+        }
+    for (int64_t d = 0; d < R; ++d)
+        for (size_t q = 0; q < SH.chan.size(); ++q) {
+            const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+            c32[d * mt * 32 + t * 32 + row] = (float)c64[(d * K + (k - 1)) * 2];
+            c32[d * mt * 32 + t * 32 + 16 + row] = (float)c64[(d * K + (k - 1)) * 2 + 1];
         }
 }
 
 void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
     if (SH.frag_off.count(b)) return;
-    const int K = N / 2, mt = (K + 15) / 16, nent = (L - b) * r, nbs = std::max(1, (nent + 15) / 16);
+    const int mt = SH.mt(), nent = (L - b) * r, nbs = std::max(1, (nent + 15) / 16);
     SH.frag_off[b] = (int64_t)SH.frags.size();
     double rho = 0;
     for (int t = 0; t < mt; ++t)
@@ -449,9 +457,10 @@ void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>
                     const int row = lane % 32;
                     const int e = 16 * (st % nbs) + 8 * (lane / 32) + i8;
                     const int part = st / nbs;
-                    const int k = 16 * t + row % 16 + 1;
+                    const size_t qs = 16 * t + row % 16;
                     uint16_t v = 0;
-                    if (e < nent && k <= K) {
+                    if (e < nent && qs < SH.chan.size()) {
+                        const int k = SH.chan[qs];
                         const int j = b + e / r, i = e % r;
                         const int p = j + L * i;
                         const int mm = (int)(((int64_t)k * p) % N);
@@ -491,8 +500,8 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
     D.role = S.seqflag ? 1 : 0;
-    const int L = S.L, r = S.r, N = S.N, K = D.K;
-    const int mt = (K + 15) / 16;
+    D.nch = (int32_t)SH.chan.size();
+    const int L = S.L, r = S.r, N = S.N;
     std::vector<int64_t> R(L);
     for (int j = 0; j < L; ++j) R[j] = (int64_t)S.R[j];
     // split: suffix [b, L) with p_suf >= 64, mid [a, b) with 16 <= p_mid <= 4096
@@ -587,6 +596,14 @@ V* upload(gp_ctx* ctx, const char* name, const std::vector<V>& v) {
 // ---------------------------------------------------------------------------
 // host: the three stages
 
+// Sides of one operand class of the split filter kernel, contiguous in table order.
+struct UncClass {
+    int nbs = 0, mt = 0;
+    bool half = false;
+    int sid_lo = 0, sid_hi = 0;
+    int64_t item_lo = 0, item_hi = 0;
+};
+
 // Host tables of one batch (built without the device, so that the next batch's
 // tables can be prepared while the current batch runs).
 struct UncPrep {
@@ -596,8 +613,33 @@ struct UncPrep {
     Tables T;
     Shared SH;
     std::vector<int> side_of;
+    std::vector<UncClass> classes;
     uint64_t total_leaves = 0;
+    uint64_t dead_leaves = 0;  // leaves of sides rejected by their fixed channels
 };
+
+template <int NBS, int MT, bool HALF>
+void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncSide* sides, const int32_t* radix,
+                     const int32_t* pos32, const float* c32, const float* mid, const uint4* sufb, const uint4* frags,
+                     UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
+    hipLaunchKernelGGL((k_unc_filter<NBS, MT, HALF>), grid, dim3(256), 0, st, sides, c.sid_lo, c.sid_hi, c.item_lo,
+                       c.item_hi, radix, pos32, c32, mid, sufb, frags, ctl, surv, scap, band, bcap);
+}
+
+void launch_filter_class(const UncClass& c, dim3 grid, hipStream_t st, const UncSide* sides, const int32_t* radix,
+                         const int32_t* pos32, const float* c32, const float* mid, const uint4* sufb,
+                         const uint4* frags, UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
+#define GP_FILTER_CASE(NB, M, H)                                                                                 \
+    if (c.nbs == NB && c.mt == M && c.half == H)                                                                 \
+        return launch_filter_t<NB, M, H>(c, grid, st, sides, radix, pos32, c32, mid, sufb, frags, ctl, surv, scap, \
+                                         band, bcap);
+    GP_FILTER_CASE(1, 1, false) GP_FILTER_CASE(2, 1, false) GP_FILTER_CASE(3, 1, false) GP_FILTER_CASE(4, 1, false)
+    GP_FILTER_CASE(1, 1, true) GP_FILTER_CASE(2, 1, true) GP_FILTER_CASE(3, 1, true) GP_FILTER_CASE(4, 1, true)
+    GP_FILTER_CASE(1, 2, false) GP_FILTER_CASE(2, 2, false) GP_FILTER_CASE(3, 2, false) GP_FILTER_CASE(4, 2, false)
+    GP_FILTER_CASE(1, 2, true) GP_FILTER_CASE(2, 2, true) GP_FILTER_CASE(3, 2, true) GP_FILTER_CASE(4, 2, true)
+#undef GP_FILTER_CASE
+    throw ArgError("uncompress: no filter kernel for this operand class");
+}
 
 UncTrig unc_trig(gp_ctx* ctx, int N) {
     if (N > kMaxN || N < 2) throw ArgError("uncompress: expanded length outside [2, 64]");
@@ -638,9 +680,42 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
     std::vector<int>& side_of = U.side_of;
     {
         Phase ph("unc_filter: side tables");
+        // Channels k = r m are fixed by the compressed word c: the expanded word x
+        // has x[j + L i] = t_j[i] with sum_i t_j[i] = c_j, so X(r m) = sum_j c_j
+        // e^{-2 pi i m j / L} for every leaf.  A side whose fixed channels all
+        // lie clearly below the threshold needs only the other channels per leaf;
+        // one clearly above it rejects every leaf; a value within 1e-6 of the
+        // threshold keeps every channel in the per-leaf filter (whole batch).
+        const int r = sides[0].r, K = N / 2;
+        const double thr = 2.0 * order - paf + 0.001;
+        bool all_channels = r <= 1 || K < r;
         std::vector<size_t> live;
-        for (size_t i = 0; i < sides.size(); ++i)
-            if (sides[i].leaves != 0) live.push_back(i);
+        for (size_t i = 0; i < sides.size(); ++i) {
+            if (sides[i].leaves == 0) continue;
+            bool dead = false;
+            for (int k = r; k <= K && r > 1; k += r) {
+                double re = 0, im = 0;
+                for (int j = 0; j < sides[i].L; ++j) {
+                    const int mm = (int)(((int64_t)k * j) % N);
+                    re += sides[i].orig[j] * cs[mm];
+                    im += sides[i].orig[j] * sn[mm];
+                }
+                const double psd = re * re + im * im;
+                if (psd > thr + 1e-6) dead = true;
+                else if (psd >= thr - 1e-6) all_channels = true;
+            }
+            if (dead) {
+                U.dead_leaves += sides[i].leaves;
+                continue;
+            }
+            live.push_back(i);
+        }
+        if (all_channels) U.dead_leaves = 0, live.clear();
+        if (all_channels)
+            for (size_t i = 0; i < sides.size(); ++i)
+                if (sides[i].leaves != 0) live.push_back(i);
+        for (int k = 1; k <= K; ++k)
+            if (all_channels || k % r != 0) SH.chan.push_back(k);
         // shared per-(position, letter) blocks and per-suffix-start fragments / weights
         std::vector<int> sa(live.size()), sb(live.size());
         Phase ph1("tables: blocks + keys");
@@ -706,7 +781,27 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             add_side(part[q], SH, sides[live[q]], rmax, order, paf, group[live[q]], fp32, mid_off[side_mid[q]],
                      suf_off[side_suf[q]]);
         });
-        for (size_t q = 0; q < live.size(); ++q) {
+        // table order: sides grouped by operand class (one kernel instantiation each)
+        auto cls = [&](size_t q) {
+            const UncSide& D = part[q].sides[0];
+            const int mt = (D.nch + 15) / 16;
+            return std::make_tuple(D.nbs, mt, D.nch <= 16 * (mt - 1) + 8);
+        };
+        std::vector<size_t> order(live.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cls(x) < cls(y); });
+        for (size_t q : order) {
+            const auto [nbs, mt, half] = cls(q);
+            if (U.classes.empty() || U.classes.back().nbs != nbs || U.classes.back().mt != mt ||
+                U.classes.back().half != half) {
+                UncClass c;
+                c.nbs = nbs;
+                c.mt = mt;
+                c.half = half;
+                c.sid_lo = c.sid_hi = (int)T.sides.size();
+                c.item_lo = c.item_hi = T.items;
+                U.classes.push_back(c);
+            }
             UncSide D = part[q].sides[0];
             D.item_base = T.items;
             D.radix_off += (int64_t)T.radix.size();
@@ -725,6 +820,8 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             side_of.push_back((int)live[q]);
             total_leaves += sides[live[q]].leaves;
             part[q] = Tables();
+            U.classes.back().sid_hi = (int)T.sides.size();
+            U.classes.back().item_hi = T.items;
         }
     }
     return prep;
@@ -742,6 +839,10 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     B.trig_off = U.trig_off;
     B.bound = U.bound;
     B.side_of = U.side_of;
+    if (stats) {  // sides rejected by their fixed channels: every leaf decided, none passed
+        stats->words += U.dead_leaves;
+        stats->psd_decisions += U.dead_leaves;
+    }
     if (T.sides.empty()) return B;
     Phase ph2("unc_filter: upload");
     B.d_sides = upload(ctx, "u_sides", T.sides);
@@ -771,14 +872,19 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         uint64_t* d_band = ctx->dev<uint64_t>("u_band", (size_t)bcap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(UncCtl), st));
         GP_HIP(hipEventRecord(ctx->ev0, st));
-        const int64_t waves = std::min<int64_t>(T.items, (int64_t)ctx->num_cus * 8);
-        const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-        if (!fp32)
-            hipLaunchKernelGGL(k_unc_filter, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items, B.d_radix,
-                               d_pos32, d_c32, d_mid, d_sufb, d_frags, ctl, d_surv, scap, d_band, bcap);
-        else
+        if (!fp32) {
+            for (const UncClass& c : U.classes) {
+                const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 8);
+                const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+                launch_filter_class(c, grid, st, B.d_sides, B.d_radix, d_pos32, d_c32, d_mid, d_sufb, d_frags, ctl,
+                                    d_surv, scap, d_band, bcap);
+            }
+        } else {
+            const int64_t waves = std::min<int64_t>(T.items, (int64_t)ctx->num_cus * 8);
+            const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
             hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items,
                                B.d_radix, d_pos32, d_c32, d_mid, d_ent, d_w32, ctl, d_surv, scap, d_band, bcap);
+        }
         GP_HIP(hipGetLastError());
         GP_HIP(hipEventRecord(ctx->ev1, st));
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
@@ -807,7 +913,7 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         stats->tie_band += h.ties;
         stats->psd_kernel_ms += ms_filter;
         stats->band_kernel_ms += ms_band;
-        stats->psd_launches += 1;
+        stats->psd_launches += fp32 ? 1 : U.classes.size();
         stats->kernel_ms += ms_filter + ms_band;
     }
     return B;
