@@ -25,13 +25,15 @@
 // ---------------------------------------------------------------------------
 // FP32 (plain) filter: one leaf per lane, VALU FMAs over the suffix entries.
 
-__global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restrict__ sides, int nsides,
-                                                         int64_t nitems, const int32_t* __restrict__ radix,
-                                                         const int32_t* __restrict__ pos32,
-                                                         const float* __restrict__ c32, const float* __restrict__ mid,
-                                                         const int8_t* __restrict__ ent, const float* __restrict__ w32,
-                                                         UncCtl* ctl, uint64_t* __restrict__ surv, int64_t scap,
+__global__ void __launch_bounds__(256) k_unc_filter_fp32(UncTabs tb, int nsides, int64_t nitems, UncCtl* ctl,
+                                                         uint64_t* __restrict__ surv, int64_t scap,
                                                          uint64_t* __restrict__ band, int64_t bcap) {
+    const UncSide* __restrict__ sides = tb.sides;
+    const int32_t* __restrict__ radix = tb.radix;
+    const float* __restrict__ c32 = tb.c32;
+    const float* __restrict__ mid = tb.mid;
+    const int8_t* __restrict__ ent = tb.ent;
+    const float* __restrict__ w32 = tb.w32;
     __shared__ float xpre[4][64];
     __shared__ uint64_t wbuf_all[4][kWaveBuf];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -47,16 +49,17 @@ __global__ void __launch_bounds__(256) k_unc_filter_fp32(const UncSide* __restri
         const int64_t o = it - S.item_base;
         const int mt = (S.nch + 15) / 16;
         const int nch = mt * 32;
-        const int nent = (S.L - S.b) * S.r;
-        // X_outer, one channel per lane
-        float xo = 0.f;
+        const int nent = S.nent;
+        // X_outer, one channel per lane: the single-split constant plus the outer digits
+        float xo = lane < nch ? tb.fix32[S.fix_off + lane] : 0.f;
         {
             int64_t rem = o;
-            for (int j = S.a - 1; j >= 0; --j) {
+            for (int x = S.nout - 1; x >= 0; --x) {
+                const int j = tb.olist[S.olist_off + x];
                 const int R = radix[S.radix_off + j];
                 const int d = (int)(rem % R);
                 rem /= R;
-                if (lane < nch) xo += c32[pos32[S.c32_off + j] + (int64_t)d * nch + lane];
+                if (lane < nch) xo += c32[tb.pos32[S.c32_off + j] + (int64_t)d * nch + lane];
             }
         }
         for (int64_t m = 0; m < S.p_mid; ++m) {
@@ -297,6 +300,8 @@ struct Tables {
     std::vector<int8_t> tup;
     std::vector<uint64_t> pm1;
     std::vector<int32_t> pos32, pos64;  // per side and position: block offsets into Shared
+    std::vector<int32_t> olist;         // per side: outer positions (active, in order)
+    std::vector<float> fix32;           // per side: FP32 sum of the single-split positions
     int64_t items = 0;
 };
 
@@ -309,32 +314,76 @@ struct Shared {
     std::vector<int32_t> off32, off64;
     std::vector<float> c32;
     std::vector<double> c64;
-    std::map<int, int64_t> frag_off;  // suffix start b -> offset into frags
-    std::map<int, double> rho;        // suffix start b -> max BF16 split residual
+    std::map<std::vector<int>, int64_t> frag_off;  // suffix positions -> offset into frags
+    std::map<std::vector<int>, double> rho;        // suffix positions -> max BF16 split residual
     std::vector<uint4> frags;
     std::vector<float> mid, w32;      // mid tables; FP32 suffix weights (FP32 path)
     std::vector<uint4> sufb;          // suffix B operands (split path)
     std::vector<int8_t> ent;          // suffix entries (FP32 path)
-    std::map<int, int64_t> w32_off;   // suffix start b -> offset into w32
+    std::map<std::vector<int>, int64_t> w32_off;  // suffix positions -> offset into w32
     // FP32 filter channels: slot q (row q % 16 of M-tile q / 16) holds k = chan[q]
     std::vector<int> chan;
     int mt() const { return ((int)chan.size() + 15) / 16; }
     int id(int j, int letter) const { return block.at({j, letter}); }
 };
 
-// Mid table of (a, b, letters): FP64 sums over the middle digits, rounded once
-// to FP32, [p_mid][mt*32] (last position fastest).
-void build_mid(const Shared& SH, const SideHost& S, int a, int b, std::vector<float>& out) {
+// The split of a side's leaf index over its active positions (more than one
+// split; single-split positions add a constant and digit 0 of radix 1, so the
+// leaf index of the DFS order is the mixed-radix number of the active digits):
+// suffix = the last active positions (p_suf >= 64, grown while the entries fit
+// the K blocks already used and p_suf stays <= 1024), mid = the active
+// positions before it (16 <= p_mid <= 4096 where possible), outer = the rest.
+struct Split {
+    std::vector<int> out, mid, suf;  // position indices, ascending
+    int64_t po = 1, pm = 1, ps = 1;
+    int nent = 0;
+};
+
+Split side_split(const SideHost& S) {
+    Split X;
+    std::vector<int> act;
+    for (int j = 0; j < S.L; ++j)
+        if (S.R[j] > 1) act.push_back(j);
+    int k = (int)act.size();
+    while (k > 0) {
+        const int j = act[k - 1];
+        if (X.ps >= 64) {
+            const int nbs_now = std::max(1, (X.nent + 15) / 16);
+            if (X.nent + S.r > 16 * nbs_now || X.ps * S.R[j] > 1024) break;
+        }
+        X.ps *= S.R[j];
+        X.nent += S.r;
+        --k;
+    }
+    X.suf.assign(act.begin() + k, act.end());
+    int a = k;
+    while (a > 0 && X.pm < 16 && X.pm * S.R[act[a - 1]] <= 4096) X.pm *= S.R[act[--a]];
+    X.mid.assign(act.begin() + a, act.begin() + k);
+    X.out.assign(act.begin(), act.begin() + a);
+    for (int j : X.out) X.po *= S.R[j];
+    return X;
+}
+
+// Content keys: the positions, then the letters at them.
+std::vector<int> pos_key(const SideHost& S, const std::vector<int>& pos) {
+    std::vector<int> k(pos);
+    for (int j : pos) k.push_back(S.orig[j]);
+    return k;
+}
+
+// Mid table of (mid positions, letters): FP64 sums over the mid digits, rounded
+// once to FP32, [p_mid][mt*32] (last position fastest).
+void build_mid(const Shared& SH, const SideHost& S, const std::vector<int>& mp, std::vector<float>& out) {
     const int K = S.N / 2, mt = SH.mt();
     int64_t pm = 1;
-    for (int j = a; j < b; ++j) pm *= (int64_t)S.R[j];
+    for (int j : mp) pm *= (int64_t)S.R[j];
     out.assign((size_t)pm * mt * 32, 0.f);
     std::vector<double> acc((size_t)2 * K);
-    std::vector<int> dg(S.L, 0);
+    std::vector<int> dg(mp.size(), 0);
     for (int64_t m = 0; m < pm; ++m) {
         std::fill(acc.begin(), acc.end(), 0.0);
-        for (int j = a; j < b; ++j) {
-            const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])] + (int64_t)dg[j] * K * 2];
+        for (size_t x = 0; x < mp.size(); ++x) {
+            const double* c = &SH.c64[SH.off64[SH.id(mp[x], S.orig[mp[x]])] + (int64_t)dg[x] * K * 2];
             for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
         }
         for (size_t q = 0; q < SH.chan.size(); ++q) {
@@ -342,27 +391,29 @@ void build_mid(const Shared& SH, const SideHost& S, int a, int b, std::vector<fl
             out[m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
             out[m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
         }
-        for (int j = b - 1; j >= a; --j) {
-            if (++dg[j] < S.R[j]) break;
-            dg[j] = 0;
+        for (int x = (int)mp.size() - 1; x >= 0; --x) {
+            if (++dg[x] < S.R[mp[x]]) break;
+            dg[x] = 0;
         }
     }
 }
 
-// Suffix operands of (b, letters): BF16 B fragments [ntiles][nbs][64] (split
-// path) or int8 entries [p_suf][nent] (FP32 path); suffix combinations in
-// mixed-radix order, last position fastest.
-void build_suf(const SideHost& S, int b, bool fp32, std::vector<uint4>& sufb, std::vector<int8_t>& ent) {
-    const int L = S.L, r = S.r, nent = (L - b) * r;
+// Suffix operands of (suffix positions, letters): BF16 B fragments
+// [ntiles][nbs][64] (split path) or int8 entries [p_suf][nent] (FP32 path).
+// Entry e = x r + i is element i of the tuple of suffix position x; suffix
+// combinations in mixed-radix order, last position fastest.
+void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, std::vector<uint4>& sufb,
+               std::vector<int8_t>& ent) {
+    const int r = S.r, nent = (int)sp.size() * r;
     int64_t ps = 1;
-    for (int j = b; j < L; ++j) ps *= (int64_t)S.R[j];
-    std::vector<int> sdg(L, 0);
-    std::vector<int8_t> vals((size_t)ps * nent);
+    for (int j : sp) ps *= (int64_t)S.R[j];
+    std::vector<int> sdg(sp.size(), 0);
+    std::vector<int8_t> vals((size_t)ps * std::max(nent, 1), 0);
     for (int64_t s = 0; s < ps; ++s) {
-        for (int e = 0; e < nent; ++e) vals[s * nent + e] = S.tuple(b + e / r, sdg[b + e / r])[e % r];
-        for (int j = L - 1; j >= b; --j) {
-            if (++sdg[j] < S.R[j]) break;
-            sdg[j] = 0;
+        for (int e = 0; e < nent; ++e) vals[s * nent + e] = S.tuple(sp[e / r], sdg[e / r])[e % r];
+        for (int x = (int)sp.size() - 1; x >= 0; --x) {
+            if (++sdg[x] < S.R[sp[x]]) break;
+            sdg[x] = 0;
         }
     }
     if (fp32) {
@@ -392,13 +443,14 @@ void build_suf(const SideHost& S, int b, bool fp32, std::vector<uint4>& sufb, st
             }
 }
 
-// FP32 suffix DFT weights of suffix start b, [nent][mt*32] (FP32 path).
-void add_w32(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
-    if (SH.w32_off.count(b)) return;
-    const int mt = SH.mt(), nent = (L - b) * r;
-    SH.w32_off[b] = (int64_t)SH.w32.size();
+// FP32 suffix DFT weights of the suffix positions, [nent][mt*32] (FP32 path).
+void add_w32(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+             const std::vector<double>& sn) {
+    if (SH.w32_off.count(sp)) return;
+    const int mt = SH.mt(), nent = (int)sp.size() * r;
+    SH.w32_off[sp] = (int64_t)SH.w32.size();
     for (int e = 0; e < nent; ++e) {
-        const int j = b + e / r, i = e % r, p = j + L * i;
+        const int j = sp[e / r], i = e % r, p = j + L * i;
         for (int t = 0; t < mt; ++t)
             for (int row = 0; row < 32; ++row) {
                 const size_t q = 16 * t + row % 16;
@@ -444,10 +496,11 @@ void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& 
         }
 }
 
-void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>& cs, const std::vector<double>& sn) {
-    if (SH.frag_off.count(b)) return;
-    const int mt = SH.mt(), nent = (L - b) * r, nbs = std::max(1, (nent + 15) / 16);
-    SH.frag_off[b] = (int64_t)SH.frags.size();
+void add_frags(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+               const std::vector<double>& sn) {
+    if (SH.frag_off.count(sp)) return;
+    const int mt = SH.mt(), nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
+    SH.frag_off[sp] = (int64_t)SH.frags.size();
     double rho = 0;
     for (int t = 0; t < mt; ++t)
         for (int st = 0; st < 2 * nbs; ++st)
@@ -461,7 +514,7 @@ void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>
                     uint16_t v = 0;
                     if (e < nent && qs < SH.chan.size()) {
                         const int k = SH.chan[qs];
-                        const int j = b + e / r, i = e % r;
+                        const int j = sp[e / r], i = e % r;
                         const int p = j + L * i;
                         const int mm = (int)(((int64_t)k * p) % N);
                         const double w = row >= 16 ? sn[mm] : cs[mm];
@@ -480,22 +533,11 @@ void add_frags(Shared& SH, int b, int L, int r, int N, const std::vector<double>
                 q.w = hv[6] | ((uint32_t)hv[7] << 16);
                 SH.frags.push_back(q);
             }
-    SH.rho[b] = rho;
+    SH.rho[sp] = rho;
 }
 
-// suffix start b and mid start a of a side (uncompression split of the leaf index)
-void side_split(const SideHost& S, int& a, int& b) {
-    const int L = S.L;
-    b = L;
-    int64_t ps = 1;
-    while (b > 0 && ps < 64) { --b; ps *= (int64_t)S.R[b]; }
-    a = b;
-    int64_t pm = 1;
-    while (a > 0 && pm < 16 && pm * (int64_t)S.R[a - 1] <= 4096) { --a; pm *= (int64_t)S.R[a]; }
-}
-
-void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int order, int paf, int group, bool fp32,
-              int64_t mid_off, int64_t suf_off) {
+void add_side(Tables& T, const Shared& SH, const SideHost& S, const Split& X, int rmax, int order, int paf,
+              int group, bool fp32, int64_t mid_off, int64_t suf_off) {
     UncSide D{};
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
@@ -504,19 +546,33 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
     const int L = S.L, r = S.r, N = S.N;
     std::vector<int64_t> R(L);
     for (int j = 0; j < L; ++j) R[j] = (int64_t)S.R[j];
-    // split: suffix [b, L) with p_suf >= 64, mid [a, b) with 16 <= p_mid <= 4096
-    int a, b;
-    side_split(S, a, b);
-    int64_t ps = 1, pm = 1, po = 1;
-    for (int j = b; j < L; ++j) ps *= R[j];
-    for (int j = a; j < b; ++j) pm *= R[j];
-    for (int j = 0; j < a; ++j) po *= R[j];
-    D.a = a; D.b = b; D.p_suf = ps; D.p_mid = pm; D.p_out = po; D.leaves = (int64_t)S.leaves;
-    const int nent = (L - b) * r;
-    D.nbs = std::max(1, (nent + 15) / 16);
+    const int64_t ps = X.ps, pm = X.pm, po = X.po;
+    D.nout = (int32_t)X.out.size();
+    D.nent = X.nent;
+    D.p_suf = ps; D.p_mid = pm; D.p_out = po; D.leaves = (int64_t)S.leaves;
+    D.nbs = std::max(1, (X.nent + 15) / 16);
     D.ntiles = (int)((ps + 31) / 32);
     D.item_base = T.items;
     T.items += po;
+    D.olist_off = (int64_t)T.olist.size();
+    for (int j : X.out) T.olist.push_back(j);
+    // single-split positions: one constant, FP64 sum rounded once to FP32
+    {
+        const int K = N / 2, mt = SH.mt();
+        std::vector<double> acc((size_t)2 * K, 0.0);
+        for (int j = 0; j < L; ++j)
+            if (R[j] == 1) {
+                const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])]];
+                for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
+            }
+        D.fix_off = (int64_t)T.fix32.size();
+        T.fix32.resize(T.fix32.size() + (size_t)mt * 32, 0.f);
+        for (size_t q = 0; q < SH.chan.size(); ++q) {
+            const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+            T.fix32[D.fix_off + t * 32 + row] = (float)acc[2 * (k - 1)];
+            T.fix32[D.fix_off + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
+        }
+    }
     D.radix_off = (int64_t)T.radix.size();
     for (int j = 0; j < L; ++j) T.radix.push_back((int32_t)R[j]);
     D.tup_off = (int64_t)T.tup.size();
@@ -552,10 +608,12 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
     D.mid_off = mid_off;
     D.sufb_off = fp32 ? 0 : suf_off;
     D.ent_off = fp32 ? suf_off : 0;
-    D.w32_off = fp32 ? SH.w32_off.at(b) : 0;
-    // A fragments (suffix DFT weights, BF16 hi and lo), shared per suffix start
-    D.frag_off = fp32 ? 0 : SH.frag_off.at(b);
-    const double rho = fp32 ? 0.0 : SH.rho.at(b);
+    D.w32_off = fp32 ? SH.w32_off.at(X.suf) : 0;
+    // A fragments (suffix DFT weights, BF16 hi and lo), shared per suffix position set
+    D.frag_off = fp32 ? 0 : SH.frag_off.at(X.suf);
+    const double rho = fp32 ? 0.0 : SH.rho.at(X.suf);
+    std::vector<char> in_suf(L, 0);
+    for (int j : X.suf) in_suf[j] = 1;
     // error bound of the filter's Re/Im
     double s1_pre = 0, s1_suf = 0;
     for (int j = 0; j < L; ++j) {
@@ -565,17 +623,17 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, int rmax, int orde
             for (int i = 0; i < r; ++i) ab += std::abs(S.tuple(j, d)[i]);
             mxabs = std::max(mxabs, ab);
         }
-        (j < b ? s1_pre : s1_suf) += mxabs;
+        (in_suf[j] ? s1_suf : s1_pre) += mxabs;
     }
     const double u24 = 0x1p-24, u23 = 0x1p-23;
     double E;
     if (!fp32)
-        E = (a + 3) * u24 * s1_pre * 1.01                                   // X_outer + Mid rounding (FP32)
+        E = (D.nout + 4) * u24 * s1_pre * 1.01                              // fixed + X_outer + Mid rounding (FP32)
             + s1_suf * rho * 1.0000001                                      // BF16 split residual
             + 1.004 * (32.0 * D.nbs + 1.0) * u23 * (s1_pre + s1_suf);       // MFMA accumulation
     else
-        E = (a + 3) * u24 * s1_pre * 1.01 + s1_suf * u24 * 1.01              // FP32 weights
-            + (nent + 1.0) * u24 * (s1_pre + s1_suf) * 1.01;                // FP32 FMAs
+        E = (D.nout + 4) * u24 * s1_pre * 1.01 + s1_suf * u24 * 1.01         // FP32 weights
+            + (X.nent + 1.0) * u24 * (s1_pre + s1_suf) * 1.01;              // FP32 FMAs
     const double thr = 2.0 * order - paf + 0.001;
     const double eps = psd_eps(E, thr);
     D.thr = thr;
@@ -619,20 +677,17 @@ struct UncPrep {
 };
 
 template <int NBS, int MT, bool HALF>
-void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncSide* sides, const int32_t* radix,
-                     const int32_t* pos32, const float* c32, const float* mid, const uint4* sufb, const uint4* frags,
-                     UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
-    hipLaunchKernelGGL((k_unc_filter<NBS, MT, HALF>), grid, dim3(256), 0, st, sides, c.sid_lo, c.sid_hi, c.item_lo,
-                       c.item_hi, radix, pos32, c32, mid, sufb, frags, ctl, surv, scap, band, bcap);
+void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl, uint64_t* surv,
+                     int64_t scap, uint64_t* band, int64_t bcap) {
+    hipLaunchKernelGGL((k_unc_filter<NBS, MT, HALF>), grid, dim3(256), 0, st, tb, c.sid_lo, c.sid_hi, c.item_lo,
+                       c.item_hi, ctl, surv, scap, band, bcap);
 }
 
-void launch_filter_class(const UncClass& c, dim3 grid, hipStream_t st, const UncSide* sides, const int32_t* radix,
-                         const int32_t* pos32, const float* c32, const float* mid, const uint4* sufb,
-                         const uint4* frags, UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
-#define GP_FILTER_CASE(NB, M, H)                                                                                 \
-    if (c.nbs == NB && c.mt == M && c.half == H)                                                                 \
-        return launch_filter_t<NB, M, H>(c, grid, st, sides, radix, pos32, c32, mid, sufb, frags, ctl, surv, scap, \
-                                         band, bcap);
+void launch_filter_class(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
+                         uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
+#define GP_FILTER_CASE(NB, M, H)                 \
+    if (c.nbs == NB && c.mt == M && c.half == H) \
+        return launch_filter_t<NB, M, H>(c, grid, st, tb, ctl, surv, scap, band, bcap);
     GP_FILTER_CASE(1, 1, false) GP_FILTER_CASE(2, 1, false) GP_FILTER_CASE(3, 1, false) GP_FILTER_CASE(4, 1, false)
     GP_FILTER_CASE(1, 1, true) GP_FILTER_CASE(2, 1, true) GP_FILTER_CASE(3, 1, true) GP_FILTER_CASE(4, 1, true)
     GP_FILTER_CASE(1, 2, false) GP_FILTER_CASE(2, 2, false) GP_FILTER_CASE(3, 2, false) GP_FILTER_CASE(4, 2, false)
@@ -716,15 +771,15 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 if (sides[i].leaves != 0) live.push_back(i);
         for (int k = 1; k <= K; ++k)
             if (all_channels || k % r != 0) SH.chan.push_back(k);
-        // shared per-(position, letter) blocks and per-suffix-start fragments / weights
-        std::vector<int> sa(live.size()), sb(live.size());
+        // shared per-(position, letter) blocks and per-suffix fragments / weights
+        std::vector<Split> spl(live.size());
         Phase ph1("tables: blocks + keys");
         for (size_t q = 0; q < live.size(); ++q) {
             const SideHost& S = sides[live[q]];
             for (int j = 0; j < S.L; ++j) add_block(SH, S, j, cs, sn);
-            side_split(S, sa[q], sb[q]);
-            if (!fp32) add_frags(SH, sb[q], S.L, S.r, N, cs, sn);
-            else add_w32(SH, sb[q], S.L, S.r, N, cs, sn);
+            spl[q] = side_split(S);
+            if (!fp32) add_frags(SH, spl[q].suf, S.L, S.r, N, cs, sn);
+            else add_w32(SH, spl[q].suf, S.L, S.r, N, cs, sn);
         }
         // content-addressed mid and suffix tables: one job per distinct key
         std::map<std::vector<int>, int> mid_job, suf_job;
@@ -732,9 +787,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         std::vector<int> side_mid(live.size()), side_suf(live.size());
         for (size_t q = 0; q < live.size(); ++q) {
             const SideHost& S = sides[live[q]];
-            std::vector<int> km{sa[q], sb[q]}, ks{sb[q]};
-            km.insert(km.end(), S.orig.begin() + sa[q], S.orig.begin() + sb[q]);
-            ks.insert(ks.end(), S.orig.begin() + sb[q], S.orig.end());
+            const std::vector<int> km = pos_key(S, spl[q].mid), ks = pos_key(S, spl[q].suf);
             auto im = mid_job.emplace(km, (int)mid_rep.size());
             if (im.second) mid_rep.push_back(q);
             side_mid[q] = im.first->second;
@@ -758,10 +811,10 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         parallel(mid_rep.size() + suf_rep.size(), [&](size_t i) {
             if (i < mid_rep.size()) {
                 const size_t q = mid_rep[i];
-                build_mid(SH, sides[live[q]], sa[q], sb[q], mids[i]);
+                build_mid(SH, sides[live[q]], spl[q].mid, mids[i]);
             } else {
                 const size_t q = suf_rep[i - mid_rep.size()];
-                build_suf(sides[live[q]], sb[q], fp32, sufbs[i - mid_rep.size()], ents[i - mid_rep.size()]);
+                build_suf(sides[live[q]], spl[q].suf, fp32, sufbs[i - mid_rep.size()], ents[i - mid_rep.size()]);
             }
         });
         std::vector<int64_t> mid_off(mids.size()), suf_off(suf_rep.size());
@@ -778,8 +831,8 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         // per-side tables on host threads, then concatenated with rebased offsets
         std::vector<Tables> part(live.size());
         parallel(live.size(), [&](size_t q) {
-            add_side(part[q], SH, sides[live[q]], rmax, order, paf, group[live[q]], fp32, mid_off[side_mid[q]],
-                     suf_off[side_suf[q]]);
+            add_side(part[q], SH, sides[live[q]], spl[q], rmax, order, paf, group[live[q]], fp32,
+                     mid_off[side_mid[q]], suf_off[side_suf[q]]);
         });
         // table order: sides grouped by operand class (one kernel instantiation each)
         auto cls = [&](size_t q) {
@@ -787,10 +840,10 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             const int mt = (D.nch + 15) / 16;
             return std::make_tuple(D.nbs, mt, D.nch <= 16 * (mt - 1) + 8);
         };
-        std::vector<size_t> order(live.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cls(x) < cls(y); });
-        for (size_t q : order) {
+        std::vector<size_t> torder(live.size());
+        std::iota(torder.begin(), torder.end(), 0);
+        std::stable_sort(torder.begin(), torder.end(), [&](size_t x, size_t y) { return cls(x) < cls(y); });
+        for (size_t q : torder) {
             const auto [nbs, mt, half] = cls(q);
             if (U.classes.empty() || U.classes.back().nbs != nbs || U.classes.back().mt != mt ||
                 U.classes.back().half != half) {
@@ -809,6 +862,8 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             D.c32_off += (int64_t)T.pos32.size();
             D.c64_off += (int64_t)T.pos64.size();
             if (D.pm1_off >= 0) D.pm1_off += (int64_t)T.pm1.size();
+            D.olist_off += (int64_t)T.olist.size();
+            D.fix_off += (int64_t)T.fix32.size();
             T.items += part[q].items;
             auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
             app(T.radix, part[q].radix);
@@ -816,6 +871,8 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             app(T.pos32, part[q].pos32);
             app(T.pos64, part[q].pos64);
             app(T.pm1, part[q].pm1);
+            app(T.olist, part[q].olist);
+            app(T.fix32, part[q].fix32);
             T.sides.push_back(D);
             side_of.push_back((int)live[q]);
             total_leaves += sides[live[q]].leaves;
@@ -852,6 +909,8 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     B.d_c64 = upload(ctx, "u_c64", SH.c64);
     B.d_pos64 = upload(ctx, "u_pos64", T.pos64);
     auto* d_pos32 = upload(ctx, "u_pos32", T.pos32);
+    auto* d_olist = upload(ctx, "u_olist", T.olist);
+    auto* d_fix32 = upload(ctx, "u_fix32", T.fix32);
     auto* d_c32 = upload(ctx, "u_c32", SH.c32);
     auto* d_mid = upload(ctx, "u_mid", SH.mid);
     auto* d_sufb = upload(ctx, "u_sufb", SH.sufb);
@@ -872,18 +931,18 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         uint64_t* d_band = ctx->dev<uint64_t>("u_band", (size_t)bcap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(UncCtl), st));
         GP_HIP(hipEventRecord(ctx->ev0, st));
+        const UncTabs tb{B.d_sides, B.d_radix, d_pos32, d_olist, d_c32, d_mid, d_fix32, d_sufb, d_frags, d_ent, d_w32};
         if (!fp32) {
             for (const UncClass& c : U.classes) {
                 const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 8);
                 const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-                launch_filter_class(c, grid, st, B.d_sides, B.d_radix, d_pos32, d_c32, d_mid, d_sufb, d_frags, ctl,
-                                    d_surv, scap, d_band, bcap);
+                launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
             }
         } else {
             const int64_t waves = std::min<int64_t>(T.items, (int64_t)ctx->num_cus * 8);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-            hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, B.d_sides, (int)T.sides.size(), T.items,
-                               B.d_radix, d_pos32, d_c32, d_mid, d_ent, d_w32, ctl, d_surv, scap, d_band, bcap);
+            hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, tb, (int)T.sides.size(), T.items, ctl, d_surv,
+                               scap, d_band, bcap);
         }
         GP_HIP(hipGetLastError());
         GP_HIP(hipEventRecord(ctx->ev1, st));
