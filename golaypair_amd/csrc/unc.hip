@@ -18,7 +18,9 @@
 #include <numeric>
 #include <tuple>
 #include <map>
+#include <stdexcept>
 #include <thread>
+#include <unordered_map>
 
 #include "unc_kernels.inc"  // opens namespace gp (closed at the end of this file)
 
@@ -310,7 +312,7 @@ struct Tables {
 // the MFMA A fragments and FP32 weights of each suffix start b, the mid tables of
 // each (a, b, letters at a..b-1) and the suffix operands of each (b, letters at b..L-1).
 struct Shared {
-    std::map<std::pair<int, int>, int> block;  // (position j, letter at j) -> block id
+    std::vector<int> block;  // [position j][letter + 128] -> block id, -1 if absent
     std::vector<int32_t> off32, off64;
     std::vector<float> c32;
     std::vector<double> c64;
@@ -324,7 +326,11 @@ struct Shared {
     // FP32 filter channels: slot q (row q % 16 of M-tile q / 16) holds k = chan[q]
     std::vector<int> chan;
     int mt() const { return ((int)chan.size() + 15) / 16; }
-    int id(int j, int letter) const { return block.at({j, letter}); }
+    int id(int j, int letter) const {
+        const int b = block[(size_t)j * 256 + (letter + 128)];
+        if (b < 0) throw std::logic_error("uncompress: missing position block");
+        return b;
+    }
 };
 
 // The split of a side's leaf index over its active positions (more than one
@@ -364,10 +370,12 @@ Split side_split(const SideHost& S) {
     return X;
 }
 
-// Content keys: the positions, then the letters at them.
-std::vector<int> pos_key(const SideHost& S, const std::vector<int>& pos) {
-    std::vector<int> k(pos);
-    for (int j : pos) k.push_back(S.orig[j]);
+// Content keys: the positions, then the letters at them, one byte each.
+std::string pos_key(const SideHost& S, const std::vector<int>& pos) {
+    std::string k;
+    k.reserve(2 * pos.size());
+    for (int j : pos) k.push_back((char)j);
+    for (int j : pos) k.push_back((char)(S.orig[j] + 128));
     return k;
 }
 
@@ -464,35 +472,42 @@ void add_w32(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const 
     }
 }
 
-void add_block(Shared& SH, const SideHost& S, int j, const std::vector<double>& cs, const std::vector<double>& sn) {
-    const auto key = std::make_pair(j, S.orig[j]);
-    if (SH.block.count(key)) return;
-    const int L = S.L, r = S.r, N = S.N, K = N / 2, mt = SH.mt();
-    const int64_t R = (int64_t)S.R[j];
-    SH.block[key] = (int)SH.off32.size();
-    SH.off32.push_back((int32_t)SH.c32.size());
-    SH.off64.push_back((int32_t)SH.c64.size());
-    SH.c32.resize(SH.c32.size() + (size_t)R * mt * 32, 0.f);
-    SH.c64.resize(SH.c64.size() + (size_t)R * 2 * K, 0.0);
-    float* c32 = &SH.c32[SH.off32.back()];
-    double* c64 = &SH.c64[SH.off64.back()];
-    for (int64_t d = 0; d < R; ++d)
-        for (int k = 1; k <= K; ++k) {
-            double re = 0, im = 0;
-            for (int i = 0; i < r; ++i) {
-                const int p = j + L * i;
-                const int mm = (int)(((int64_t)k * p) % N);
-                re += S.tuple(j, d)[i] * cs[mm];
-                im += S.tuple(j, d)[i] * sn[mm];
-            }
-            c64[(d * K + (k - 1)) * 2] = re;
-            c64[(d * K + (k - 1)) * 2 + 1] = im;
-        }
-    for (int64_t d = 0; d < R; ++d)
-        for (size_t q = 0; q < SH.chan.size(); ++q) {
-            const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
-            c32[d * mt * 32 + t * 32 + row] = (float)c64[(d * K + (k - 1)) * 2];
-            c32[d * mt * 32 + t * 32 + 16 + row] = (float)c64[(d * K + (k - 1)) * 2 + 1];
+// Blocks of every position j < L and every letter of the split table: the DFT
+// contributions of each of the position's tuples (FP64 (Re, Im) for all k, and
+// the FP32 filter channels).
+void add_blocks(Shared& SH, const SplitTable& X, int L, int N, const std::vector<double>& cs,
+                const std::vector<double>& sn) {
+    const int r = X.r, K = N / 2, mt = SH.mt();
+    SH.block.assign((size_t)L * 256, -1);
+    for (int j = 0; j < L; ++j)
+        for (const auto& [letter, lid] : X.letter_id) {
+            const int R = j == 0 ? X.nfirst[lid] : X.nsplit[lid];
+            const int8_t* tp = j == 0 ? X.first[lid].data() : X.split[lid].data();
+            SH.block[(size_t)j * 256 + (letter + 128)] = (int)SH.off32.size();
+            SH.off32.push_back((int32_t)SH.c32.size());
+            SH.off64.push_back((int32_t)SH.c64.size());
+            SH.c32.resize(SH.c32.size() + (size_t)std::max(R, 1) * mt * 32, 0.f);
+            SH.c64.resize(SH.c64.size() + (size_t)std::max(R, 1) * 2 * K, 0.0);
+            float* c32 = &SH.c32[SH.off32.back()];
+            double* c64 = &SH.c64[SH.off64.back()];
+            for (int d = 0; d < R; ++d)
+                for (int k = 1; k <= K; ++k) {
+                    double re = 0, im = 0;
+                    for (int i = 0; i < r; ++i) {
+                        const int p = j + L * i;
+                        const int mm = (int)(((int64_t)k * p) % N);
+                        re += tp[d * r + i] * cs[mm];
+                        im += tp[d * r + i] * sn[mm];
+                    }
+                    c64[((int64_t)d * K + (k - 1)) * 2] = re;
+                    c64[((int64_t)d * K + (k - 1)) * 2 + 1] = im;
+                }
+            for (int d = 0; d < R; ++d)
+                for (size_t q = 0; q < SH.chan.size(); ++q) {
+                    const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+                    c32[(int64_t)d * mt * 32 + t * 32 + row] = (float)c64[((int64_t)d * K + (k - 1)) * 2];
+                    c32[(int64_t)d * mt * 32 + t * 32 + 16 + row] = (float)c64[((int64_t)d * K + (k - 1)) * 2 + 1];
+                }
         }
 }
 
@@ -536,74 +551,78 @@ void add_frags(Shared& SH, const std::vector<int>& sp, int L, int r, int N, cons
     SH.rho[sp] = rho;
 }
 
-void add_side(Tables& T, const Shared& SH, const SideHost& S, const Split& X, int rmax, int order, int paf,
-              int group, bool fp32, int64_t mid_off, int64_t suf_off) {
+// Where one side's rows go in the batch tables (filled by add_side on host threads).
+struct SideSlot {
+    int sid = 0;
+    bool pm1 = false;
+    int64_t item_base = 0, radix_off = 0, tup_off = 0, pm1_off = -1, pos_off = 0, olist_off = 0, fix_off = 0;
+};
+
+// Whether every tuple entry of the side is +-1 (words then carry +1 bit masks).
+bool side_pm1(const SideHost& S) {
+    if (S.N > 64) return false;
+    for (int j = 0; j < S.L; ++j)
+        for (int d = 0; d < S.R[j]; ++d)
+            for (int i = 0; i < S.r; ++i)
+                if (S.tuple(j, d)[i] != 1 && S.tuple(j, d)[i] != -1) return false;
+    return true;
+}
+
+void add_side(Tables& T, const SideSlot& Z, const Shared& SH, const SideHost& S, const Split& X, int rmax,
+              int order, int paf, int group, bool fp32, int64_t mid_off, int64_t suf_off) {
     UncSide D{};
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
     D.role = S.seqflag ? 1 : 0;
     D.nch = (int32_t)SH.chan.size();
     const int L = S.L, r = S.r, N = S.N;
-    std::vector<int64_t> R(L);
-    for (int j = 0; j < L; ++j) R[j] = (int64_t)S.R[j];
     const int64_t ps = X.ps, pm = X.pm, po = X.po;
     D.nout = (int32_t)X.out.size();
     D.nent = X.nent;
     D.p_suf = ps; D.p_mid = pm; D.p_out = po; D.leaves = (int64_t)S.leaves;
     D.nbs = std::max(1, (X.nent + 15) / 16);
     D.ntiles = (int)((ps + 31) / 32);
-    D.item_base = T.items;
-    T.items += po;
-    D.olist_off = (int64_t)T.olist.size();
-    for (int j : X.out) T.olist.push_back(j);
+    D.item_base = Z.item_base;
+    D.olist_off = Z.olist_off;
+    for (size_t x = 0; x < X.out.size(); ++x) T.olist[Z.olist_off + x] = X.out[x];
     // single-split positions: one constant, FP64 sum rounded once to FP32
     {
-        const int K = N / 2, mt = SH.mt();
-        std::vector<double> acc((size_t)2 * K, 0.0);
+        const int K = N / 2;
+        double acc[2 * kMaxK] = {};
         for (int j = 0; j < L; ++j)
-            if (R[j] == 1) {
+            if (S.R[j] == 1) {
                 const double* c = &SH.c64[SH.off64[SH.id(j, S.orig[j])]];
                 for (int q = 0; q < 2 * K; ++q) acc[q] += c[q];
             }
-        D.fix_off = (int64_t)T.fix32.size();
-        T.fix32.resize(T.fix32.size() + (size_t)mt * 32, 0.f);
+        D.fix_off = Z.fix_off;
         for (size_t q = 0; q < SH.chan.size(); ++q) {
             const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
             T.fix32[D.fix_off + t * 32 + row] = (float)acc[2 * (k - 1)];
             T.fix32[D.fix_off + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
         }
     }
-    D.radix_off = (int64_t)T.radix.size();
-    for (int j = 0; j < L; ++j) T.radix.push_back((int32_t)R[j]);
-    D.tup_off = (int64_t)T.tup.size();
-    T.tup.resize(T.tup.size() + (size_t)L * rmax * r, 0);
-    bool pm1_ok = N <= 64;
+    D.radix_off = Z.radix_off;
+    for (int j = 0; j < L; ++j) T.radix[Z.radix_off + j] = S.R[j];
+    D.tup_off = Z.tup_off;
     for (int j = 0; j < L; ++j)
-        for (int64_t d = 0; d < R[j]; ++d)
-            for (int i = 0; i < r; ++i) {
-                const int v = S.tuple(j, d)[i];
-                T.tup[D.tup_off + ((int64_t)j * rmax + d) * r + i] = (int8_t)v;
-                if (v != 1 && v != -1) pm1_ok = false;
-            }
-    D.pm1_off = -1;
-    if (pm1_ok) {
-        D.pm1_off = (int64_t)T.pm1.size();
-        T.pm1.resize(T.pm1.size() + (size_t)L * rmax, 0);
+        for (int d = 0; d < S.R[j]; ++d)
+            for (int i = 0; i < r; ++i) T.tup[D.tup_off + ((int64_t)j * rmax + d) * r + i] = S.tuple(j, d)[i];
+    D.pm1_off = Z.pm1 ? Z.pm1_off : -1;
+    if (Z.pm1)
         for (int j = 0; j < L; ++j)
-            for (int64_t d = 0; d < R[j]; ++d) {
+            for (int d = 0; d < S.R[j]; ++d) {
                 uint64_t msk = 0;
                 for (int i = 0; i < r; ++i)
                     if (S.tuple(j, d)[i] == 1) msk |= 1ull << (j + L * i);
                 T.pm1[D.pm1_off + (int64_t)j * rmax + d] = msk;
             }
-    }
     // per-position contribution blocks (shared between sides)
-    D.c32_off = (int64_t)T.pos32.size();
-    D.c64_off = (int64_t)T.pos64.size();
+    D.c32_off = Z.pos_off;
+    D.c64_off = Z.pos_off;
     for (int j = 0; j < L; ++j) {
         const int id = SH.id(j, S.orig[j]);
-        T.pos32.push_back(SH.off32[id]);
-        T.pos64.push_back(SH.off64[id]);
+        T.pos32[Z.pos_off + j] = SH.off32[id];
+        T.pos64[Z.pos_off + j] = SH.off64[id];
     }
     D.mid_off = mid_off;
     D.sufb_off = fp32 ? 0 : suf_off;
@@ -618,7 +637,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, const Split& X, in
     double s1_pre = 0, s1_suf = 0;
     for (int j = 0; j < L; ++j) {
         double mxabs = 0;
-        for (int64_t d = 0; d < R[j]; ++d) {
+        for (int d = 0; d < S.R[j]; ++d) {
             double ab = 0;
             for (int i = 0; i < r; ++i) ab += std::abs(S.tuple(j, d)[i]);
             mxabs = std::max(mxabs, ab);
@@ -639,7 +658,7 @@ void add_side(Tables& T, const Shared& SH, const SideHost& S, const Split& X, in
     D.thr = thr;
     D.thr_hi = std::nextafter((float)(thr + eps), INFINITY);
     D.thr_lo = std::nextafter((float)(thr - eps), -INFINITY);
-    T.sides.push_back(D);
+    T.sides[Z.sid] = D;
 }
 
 template <class V>
@@ -771,30 +790,6 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 if (sides[i].leaves != 0) live.push_back(i);
         for (int k = 1; k <= K; ++k)
             if (all_channels || k % r != 0) SH.chan.push_back(k);
-        // shared per-(position, letter) blocks and per-suffix fragments / weights
-        std::vector<Split> spl(live.size());
-        Phase ph1("tables: blocks + keys");
-        for (size_t q = 0; q < live.size(); ++q) {
-            const SideHost& S = sides[live[q]];
-            for (int j = 0; j < S.L; ++j) add_block(SH, S, j, cs, sn);
-            spl[q] = side_split(S);
-            if (!fp32) add_frags(SH, spl[q].suf, S.L, S.r, N, cs, sn);
-            else add_w32(SH, spl[q].suf, S.L, S.r, N, cs, sn);
-        }
-        // content-addressed mid and suffix tables: one job per distinct key
-        std::map<std::vector<int>, int> mid_job, suf_job;
-        std::vector<size_t> mid_rep, suf_rep;  // representative side (index into live) per job
-        std::vector<int> side_mid(live.size()), side_suf(live.size());
-        for (size_t q = 0; q < live.size(); ++q) {
-            const SideHost& S = sides[live[q]];
-            const std::vector<int> km = pos_key(S, spl[q].mid), ks = pos_key(S, spl[q].suf);
-            auto im = mid_job.emplace(km, (int)mid_rep.size());
-            if (im.second) mid_rep.push_back(q);
-            side_mid[q] = im.first->second;
-            auto is = suf_job.emplace(ks, (int)suf_rep.size());
-            if (is.second) suf_rep.push_back(q);
-            side_suf[q] = is.first->second;
-        }
         const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
         auto parallel = [nt](size_t count, const std::function<void(size_t)>& f) {
             std::vector<std::thread> th;
@@ -804,6 +799,39 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 });
             for (auto& x : th) x.join();
         };
+        // shared per-(position, letter) blocks; per side its split and table keys
+        Phase ph1("tables: blocks + keys");
+        if (!live.empty()) add_blocks(SH, *sides[live[0]].tab, sides[live[0]].L, N, cs, sn);
+        std::vector<Split> spl(live.size());
+        std::vector<std::string> kmid(live.size()), ksuf(live.size());
+        std::vector<char> pm1(live.size());
+        parallel(live.size(), [&](size_t q) {
+            const SideHost& S = sides[live[q]];
+            spl[q] = side_split(S);
+            kmid[q] = pos_key(S, spl[q].mid);
+            ksuf[q] = pos_key(S, spl[q].suf);
+            pm1[q] = side_pm1(S);
+        });
+        for (size_t q = 0; q < live.size(); ++q) {
+            const SideHost& S = sides[live[q]];
+            if (!fp32) add_frags(SH, spl[q].suf, S.L, S.r, N, cs, sn);
+            else add_w32(SH, spl[q].suf, S.L, S.r, N, cs, sn);
+        }
+        // content-addressed mid and suffix tables: one job per distinct key
+        std::unordered_map<std::string, int> mid_job, suf_job;
+        std::vector<size_t> mid_rep, suf_rep;  // representative side (index into live) per job
+        std::vector<int> side_mid(live.size()), side_suf(live.size());
+        for (size_t q = 0; q < live.size(); ++q) {
+            auto im = mid_job.emplace(kmid[q], (int)mid_rep.size());
+            if (im.second) mid_rep.push_back(q);
+            side_mid[q] = im.first->second;
+            auto is = suf_job.emplace(ksuf[q], (int)suf_rep.size());
+            if (is.second) suf_rep.push_back(q);
+            side_suf[q] = is.first->second;
+        }
+        if (std::getenv("GOLAY_PROFILE"))
+            std::fprintf(stderr, "[golay] prep: %zu sides, %zu mid tables, %zu suffix tables, %zu fragment sets\n",
+                         live.size(), mid_rep.size(), suf_rep.size(), SH.frag_off.size() + SH.w32_off.size());
         ph1.next("tables: mid/suffix jobs");
         std::vector<std::vector<float>> mids(mid_rep.size());
         std::vector<std::vector<uint4>> sufbs(suf_rep.size());
@@ -827,59 +855,65 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             SH.sufb.insert(SH.sufb.end(), sufbs[i].begin(), sufbs[i].end());
             SH.ent.insert(SH.ent.end(), ents[i].begin(), ents[i].end());
         }
-        ph1.next("tables: sides + merge");
-        // per-side tables on host threads, then concatenated with rebased offsets
-        std::vector<Tables> part(live.size());
-        parallel(live.size(), [&](size_t q) {
-            add_side(part[q], SH, sides[live[q]], spl[q], rmax, order, paf, group[live[q]], fp32,
-                     mid_off[side_mid[q]], suf_off[side_suf[q]]);
-        });
+        ph1.next("tables: sides");
         // table order: sides grouped by operand class (one kernel instantiation each)
-        auto cls = [&](size_t q) {
-            const UncSide& D = part[q].sides[0];
-            const int mt = (D.nch + 15) / 16;
-            return std::make_tuple(D.nbs, mt, D.nch <= 16 * (mt - 1) + 8);
-        };
+        const int mtc = SH.mt();
+        const bool halfc = (int)SH.chan.size() <= 16 * (mtc - 1) + 8;
+        auto cls = [&](size_t q) { return std::max(1, (spl[q].nent + 15) / 16); };
         std::vector<size_t> torder(live.size());
         std::iota(torder.begin(), torder.end(), 0);
         std::stable_sort(torder.begin(), torder.end(), [&](size_t x, size_t y) { return cls(x) < cls(y); });
-        for (size_t q : torder) {
-            const auto [nbs, mt, half] = cls(q);
-            if (U.classes.empty() || U.classes.back().nbs != nbs || U.classes.back().mt != mt ||
-                U.classes.back().half != half) {
+        std::vector<SideSlot> slot(live.size());
+        int64_t n_radix = 0, n_tup = 0, n_pm1 = 0, n_pos = 0, n_olist = 0, n_fix = 0;
+        for (size_t x = 0; x < torder.size(); ++x) {
+            const size_t q = torder[x];
+            const SideHost& S = sides[live[q]];
+            const int nbs = cls(q);
+            if (U.classes.empty() || U.classes.back().nbs != nbs) {
                 UncClass c;
                 c.nbs = nbs;
-                c.mt = mt;
-                c.half = half;
-                c.sid_lo = c.sid_hi = (int)T.sides.size();
+                c.mt = mtc;
+                c.half = halfc;
+                c.sid_lo = c.sid_hi = (int)x;
                 c.item_lo = c.item_hi = T.items;
                 U.classes.push_back(c);
             }
-            UncSide D = part[q].sides[0];
-            D.item_base = T.items;
-            D.radix_off += (int64_t)T.radix.size();
-            D.tup_off += (int64_t)T.tup.size();
-            D.c32_off += (int64_t)T.pos32.size();
-            D.c64_off += (int64_t)T.pos64.size();
-            if (D.pm1_off >= 0) D.pm1_off += (int64_t)T.pm1.size();
-            D.olist_off += (int64_t)T.olist.size();
-            D.fix_off += (int64_t)T.fix32.size();
-            T.items += part[q].items;
-            auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
-            app(T.radix, part[q].radix);
-            app(T.tup, part[q].tup);
-            app(T.pos32, part[q].pos32);
-            app(T.pos64, part[q].pos64);
-            app(T.pm1, part[q].pm1);
-            app(T.olist, part[q].olist);
-            app(T.fix32, part[q].fix32);
-            T.sides.push_back(D);
+            SideSlot& Z = slot[q];
+            Z.sid = (int)x;
+            Z.item_base = T.items;
+            T.items += spl[q].po;
+            Z.radix_off = n_radix;
+            n_radix += S.L;
+            Z.tup_off = n_tup;
+            n_tup += (int64_t)S.L * rmax * S.r;
+            Z.pm1 = pm1[q];
+            if (Z.pm1) {
+                Z.pm1_off = n_pm1;
+                n_pm1 += (int64_t)S.L * rmax;
+            }
+            Z.pos_off = n_pos;
+            n_pos += S.L;
+            Z.olist_off = n_olist;
+            n_olist += (int64_t)spl[q].out.size();
+            Z.fix_off = n_fix;
+            n_fix += (int64_t)mtc * 32;
             side_of.push_back((int)live[q]);
-            total_leaves += sides[live[q]].leaves;
-            part[q] = Tables();
-            U.classes.back().sid_hi = (int)T.sides.size();
+            total_leaves += S.leaves;
+            U.classes.back().sid_hi = (int)x + 1;
             U.classes.back().item_hi = T.items;
         }
+        T.sides.resize(live.size());
+        T.radix.assign(n_radix, 0);
+        T.tup.assign(n_tup, 0);
+        T.pm1.assign(n_pm1, 0);
+        T.pos32.assign(n_pos, 0);
+        T.pos64.assign(n_pos, 0);
+        T.olist.assign(n_olist, 0);
+        T.fix32.assign(n_fix, 0.f);
+        parallel(live.size(), [&](size_t q) {
+            add_side(T, slot[q], SH, sides[live[q]], spl[q], rmax, order, paf, group[live[q]], fp32,
+                     mid_off[side_mid[q]], suf_off[side_suf[q]]);
+        });
     }
     return prep;
 }
