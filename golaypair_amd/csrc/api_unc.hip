@@ -138,26 +138,12 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         };
         std::future<std::shared_ptr<UncPrep>> next;
         if (!batches.empty()) next = std::async(std::launch::async, prepare, 0);
-        for (size_t bi = 0; bi < batches.size(); ++bi) {
-            const auto [q0, q1] = batches[bi];
-            std::shared_ptr<UncPrep> prep = next.get();
-            if (bi + 1 < batches.size()) next = std::async(std::launch::async, prepare, bi + 1);
-            UncBatch B;
-            {
-                Phase ph("uncompress: filter batch");
-                B = unc_launch(ctx, *prep, stats);
-            }
-            UncEmitted E;
-            {
-                Phase ph("uncompress: join batch");
-                E = unc_join(ctx, B, paf * new_compress, stats);
-            }
-            Phase ph_order("uncompress: order + tmp");
-            if (q1 == P && (tmp_a || tmp_b)) {  // the last pair's candidate files
-                last_a = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0)));
-                last_b = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0) + 1));
-            }
-            // per input pair: emission order (A line, B line) in text order
+        // Emitted pairs of one batch in the reference's order (input pair, then A
+        // line text, then B line text), appended to `result`.  Runs on a host
+        // thread while the device works on the next batch; tasks are chained so
+        // batches append in order.
+        uint64_t ord_ties = 0, ord_pairs = 0;
+        auto order_batch = [&, N](UncEmitted& E) {
             const int K = N / 2;
             std::string ta, tb;
             std::vector<size_t> oa, ob;
@@ -174,7 +160,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             for (int64_t i = 0; i < E.count; ++i) {
                 // key texts must agree (they do unless an FP64 PSD sits on a rounding tie)
                 if (std::memcmp(&E.keys[i * K], &E.keys[(E.count + i) * K], K) != 0) {
-                    if (stats) stats->tie_band += 1;
+                    ++ord_ties;
                     continue;
                 }
                 ord.push_back(i);
@@ -190,7 +176,37 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 result.a.insert(result.a.end(), &E.words[i * N], &E.words[i * N] + N);
                 result.b.insert(result.b.end(), &E.words[(E.count + i) * N], &E.words[(E.count + i) * N] + N);
             }
-            if (stats) stats->pairs += ord.size();
+            ord_pairs += ord.size();
+        };
+        std::future<void> ordered;
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            const auto [q0, q1] = batches[bi];
+            std::shared_ptr<UncPrep> prep = next.get();
+            if (bi + 1 < batches.size()) next = std::async(std::launch::async, prepare, bi + 1);
+            UncBatch B;
+            {
+                Phase ph("uncompress: filter batch");
+                B = unc_launch(ctx, *prep, stats);
+            }
+            auto E = std::make_shared<UncEmitted>();
+            {
+                Phase ph("uncompress: join batch");
+                *E = unc_join(ctx, B, paf * new_compress, stats);
+            }
+            if (q1 == P && (tmp_a || tmp_b)) {  // the last pair's candidate files
+                last_a = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0)));
+                last_b = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0) + 1));
+            }
+            ordered = std::async(std::launch::async, [&order_batch, E, prev = std::move(ordered)]() mutable {
+                if (prev.valid()) prev.get();
+                Phase ph("uncompress: order batch");
+                order_batch(*E);
+            });
+        }
+        if (ordered.valid()) ordered.get();
+        if (stats) {
+            stats->tie_band += ord_ties;
+            stats->pairs += ord_pairs;
         }
         int rc = GP_OK;
         if (P > 0) {

@@ -381,11 +381,11 @@ std::string pos_key(const SideHost& S, const std::vector<int>& pos) {
 
 // Mid table of (mid positions, letters): FP64 sums over the mid digits, rounded
 // once to FP32, [p_mid][mt*32] (last position fastest).
-void build_mid(const Shared& SH, const SideHost& S, const std::vector<int>& mp, std::vector<float>& out) {
+void build_mid(const Shared& SH, const SideHost& S, const std::vector<int>& mp, float* out) {
     const int K = S.N / 2, mt = SH.mt();
     int64_t pm = 1;
     for (int j : mp) pm *= (int64_t)S.R[j];
-    out.assign((size_t)pm * mt * 32, 0.f);
+    std::fill(out, out + (size_t)pm * mt * 32, 0.f);
     std::vector<double> acc((size_t)2 * K);
     std::vector<int> dg(mp.size(), 0);
     for (int64_t m = 0; m < pm; ++m) {
@@ -410,8 +410,7 @@ void build_mid(const Shared& SH, const SideHost& S, const std::vector<int>& mp, 
 // [ntiles][nbs][64] (split path) or int8 entries [p_suf][nent] (FP32 path).
 // Entry e = x r + i is element i of the tuple of suffix position x; suffix
 // combinations in mixed-radix order, last position fastest.
-void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, std::vector<uint4>& sufb,
-               std::vector<int8_t>& ent) {
+void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, uint4* sufb, int8_t* ent) {
     const int r = S.r, nent = (int)sp.size() * r;
     int64_t ps = 1;
     for (int j : sp) ps *= (int64_t)S.R[j];
@@ -425,13 +424,11 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, std::ve
         }
     }
     if (fp32) {
-        ent = std::move(vals);
+        std::copy(vals.begin(), vals.end(), ent);
         return;
     }
     const int nbs = std::max(1, (nent + 15) / 16);
     const int ntiles = (int)((ps + 31) / 32);
-    sufb.clear();
-    sufb.reserve((size_t)ntiles * nbs * 64);
     for (int tile = 0; tile < ntiles; ++tile)
         for (int blk = 0; blk < nbs; ++blk)
             for (int lane = 0; lane < 64; ++lane) {
@@ -447,8 +444,15 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, std::ve
                 q.y = hv[2] | ((uint32_t)hv[3] << 16);
                 q.z = hv[4] | ((uint32_t)hv[5] << 16);
                 q.w = hv[6] | ((uint32_t)hv[7] << 16);
-                sufb.push_back(q);
+                *sufb++ = q;
             }
+}
+
+// Element counts of the mid and suffix tables of a split (their layouts above).
+int64_t mid_size(const Split& X, int mt) { return X.pm * mt * 32; }
+int64_t suf_size(const Split& X, bool fp32) {
+    if (fp32) return X.ps * std::max(X.nent, 1);
+    return ((X.ps + 31) / 32) * std::max(1, (X.nent + 15) / 16) * 64;
 }
 
 // FP32 suffix DFT weights of the suffix positions, [nent][mt*32] (FP32 path).
@@ -833,28 +837,30 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             std::fprintf(stderr, "[golay] prep: %zu sides, %zu mid tables, %zu suffix tables, %zu fragment sets\n",
                          live.size(), mid_rep.size(), suf_rep.size(), SH.frag_off.size() + SH.w32_off.size());
         ph1.next("tables: mid/suffix jobs");
-        std::vector<std::vector<float>> mids(mid_rep.size());
-        std::vector<std::vector<uint4>> sufbs(suf_rep.size());
-        std::vector<std::vector<int8_t>> ents(suf_rep.size());
+        // offsets first, then every table is built in place on host threads
+        std::vector<int64_t> mid_off(mid_rep.size()), suf_off(suf_rep.size());
+        int64_t n_mid = 0, n_suf = 0;
+        for (size_t i = 0; i < mid_rep.size(); ++i) {
+            mid_off[i] = n_mid;
+            n_mid += mid_size(spl[mid_rep[i]], SH.mt());
+        }
+        for (size_t i = 0; i < suf_rep.size(); ++i) {
+            suf_off[i] = n_suf;
+            n_suf += suf_size(spl[suf_rep[i]], fp32);
+        }
+        SH.mid.resize(n_mid);
+        if (fp32) SH.ent.resize(n_suf);
+        else SH.sufb.resize(n_suf);
         parallel(mid_rep.size() + suf_rep.size(), [&](size_t i) {
             if (i < mid_rep.size()) {
                 const size_t q = mid_rep[i];
-                build_mid(SH, sides[live[q]], spl[q].mid, mids[i]);
+                build_mid(SH, sides[live[q]], spl[q].mid, SH.mid.data() + mid_off[i]);
             } else {
-                const size_t q = suf_rep[i - mid_rep.size()];
-                build_suf(sides[live[q]], spl[q].suf, fp32, sufbs[i - mid_rep.size()], ents[i - mid_rep.size()]);
+                const size_t k = i - mid_rep.size(), q = suf_rep[k];
+                build_suf(sides[live[q]], spl[q].suf, fp32, fp32 ? nullptr : SH.sufb.data() + suf_off[k],
+                          fp32 ? SH.ent.data() + suf_off[k] : nullptr);
             }
         });
-        std::vector<int64_t> mid_off(mids.size()), suf_off(suf_rep.size());
-        for (size_t i = 0; i < mids.size(); ++i) {
-            mid_off[i] = (int64_t)SH.mid.size();
-            SH.mid.insert(SH.mid.end(), mids[i].begin(), mids[i].end());
-        }
-        for (size_t i = 0; i < suf_rep.size(); ++i) {
-            suf_off[i] = fp32 ? (int64_t)SH.ent.size() : (int64_t)SH.sufb.size();
-            SH.sufb.insert(SH.sufb.end(), sufbs[i].begin(), sufbs[i].end());
-            SH.ent.insert(SH.ent.end(), ents[i].begin(), ents[i].end());
-        }
         ph1.next("tables: sides");
         // table order: sides grouped by operand class (one kernel instantiation each)
         const int mtc = SH.mt();

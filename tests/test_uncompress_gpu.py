@@ -87,3 +87,41 @@ def test_uncompress_compress1_is_noop(gpu_ctx):
     a = np.ones((2, 16), np.int8)
     oa, ob, _, _, _ = gpu_ctx.uncompress(16, 1, 1, 0, a, a)
     assert len(oa) == 0
+
+
+@pytest.mark.parametrize("order,compress,paf,npairs", [(90, [10, 5], 0, 3), (45, [3, 1], -2, 2)])
+def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs):
+    """Matched pairs of a full-size level (v=90 c=10 -> 5, filtered at length 18;
+    v=45 c=3 -> 1, length 45): GPU gen/sort/match, the first `npairs` input
+    pairs that emit pairs at the finer level (found one pair at a time on the
+    GPU) plus the pair before the first of them, then uncompress and the
+    equivalence filter on the GPU and in the oracle over that slice."""
+    c, c2 = compress
+    a, b, _ = gpu_ctx.generate(order, c, paf)
+    rc, pa, pb, _ = gpu_ctx.match(order, c, paf, gpu_ctx.sort_candidates(a), gpu_ctx.sort_candidates(b))
+    assert rc == 0
+    pick = []
+    for i in range(min(len(pa), 400)):
+        oa, _, _, _, _ = gpu_ctx.uncompress(order, c, c2, paf, pa[i:i + 1], pb[i:i + 1])
+        if len(oa):
+            if not pick and i > 0:
+                pick.append(i - 1)  # an input pair that emits nothing
+            pick.append(i)
+            if len(pick) > npairs:
+                break
+    assert len(pick) >= npairs
+    pa, pb = pa[pick], pb[pick]
+    d = tempfile.mkdtemp()
+    wd = os.path.join(d, "order-%d" % order)
+    os.makedirs(wd)
+    open(os.path.join(wd, "%d-pairs-%d" % (order, c)), "w").write(
+        "".join(g.pair_line(x, y) + "\n" for x, y in zip(pa, pb)))
+    subprocess.run([_oracle.CLI, str(order), "-c", str(c), str(c2), "--paf", str(paf), "-d", d, "uncompress",
+                    "filter"], check=True, capture_output=True, timeout=600)
+    oa, ob, _, _, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb)
+    assert st.tie_band == 0 and len(oa) > 0
+    assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)) == \
+        open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
+    fa, fb, _ = gpu_ctx.filter(order, c2, oa, ob)
+    assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(fa, fb)) == \
+        open(os.path.join(wd, "%d-unique" % order)).read()
