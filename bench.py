@@ -136,6 +136,72 @@ def copy_probe(mib=1024, reps=4):
     print(json.dumps({"copy_probe_bytes": mib << 20, "reps": reps}))
 
 
+E2E_CONFIGS = {  # name -> CLI arguments (the configurations of the reference's workloads)
+    "v16": ["16", "--compress", "2", "1"],
+    "v45": ["45", "--paf", "-2", "--compress", "3", "1"],
+    "v90": ["90", "--compress", "10", "5"],
+    "v50": ["50", "--compress", "5", "1"],
+}
+
+
+def e2e(names, workdir="/tmp/golay_e2e"):
+    """End-to-end search time: wall clock of the drop-in CLI from start until
+    {v}-unique is written (every results file written), per configuration."""
+    import hashlib
+    import shutil
+    import subprocess
+    cli = os.path.join(ROOT, "golaypair_amd", "bin", "golaypair")
+    out = {}
+    for name in names:
+        args = E2E_CONFIGS[name]
+        d = os.path.join(workdir, name)
+        shutil.rmtree(d, ignore_errors=True)
+        print("e2e %s: %s" % (name, " ".join(args)), file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        p = subprocess.run([cli] + args + ["-d", d, "all"], capture_output=True, text=True)
+        dt = time.perf_counter() - t0
+        uniq = os.path.join(d, "order-" + args[0], args[0] + "-unique")
+        r = {"args": " ".join(args + ["-d", d, "all"]), "rc": p.returncode, "seconds": dt}
+        if os.path.exists(uniq):
+            data = open(uniq, "rb").read()
+            r["unique_classes"] = data.count(b"\n")
+            r["unique_sha256"] = hashlib.sha256(data).hexdigest()
+        out[name] = r
+    return out
+
+
+def fp32_parity(order, c0, c1, paf, chunk=200):
+    """The uncompression of every matched pair of a level on both PSD paths (in
+    chunks of pairs, printing progress): outputs must be identical; reports the
+    kernel-time ratio FP32 / split-BF16."""
+    import hashlib
+    import golaypair_amd as g
+    ctx = g.Context(0)
+    a, b, _ = ctx.generate(order, c0, paf)
+    rc, pa, pb, _ = ctx.match(order, c0, paf, ctx.sort_candidates(a), ctx.sort_candidates(b))
+    res = {"order": order, "compress": [c0, c1], "paf": paf, "pairs_in": len(pa)}
+    texts = {}
+    for mode, name in ((g.GP_PSD_SPLIT_BF16, "split_bf16"), (g.GP_PSD_FP32, "fp32")):
+        ctx.set_mode(mode)
+        parts, leaves, kern, t0 = [], 0, 0.0, time.perf_counter()
+        for q in range(0, len(pa), chunk):
+            oa, ob, _, _, st = ctx.uncompress(order, c0, c1, paf, pa[q:q + chunk], pb[q:q + chunk])
+            parts.append("".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)))
+            leaves += st.psd_decisions
+            kern += st.psd_kernel_ms
+            print("%s: %d/%d pairs, %.3e leaves, %.1f s" % (name, min(q + chunk, len(pa)), len(pa), leaves,
+                                                           time.perf_counter() - t0), file=sys.stderr, flush=True)
+        texts[name] = "".join(parts)
+        res[name] = {"leaves": leaves, "psd_kernel_ms": kern, "wall_s": time.perf_counter() - t0,
+                     "pairs_out": texts[name].count("\n"),
+                     "sha256": hashlib.sha256(texts[name].encode()).hexdigest(),
+                     "candidates_per_s_kernel": leaves / (kern * 1e-3)}
+    ctx.close()
+    res["identical"] = texts["split_bf16"] == texts["fp32"]
+    res["fp32_over_split_kernel_time"] = res["fp32"]["psd_kernel_ms"] / res["split_bf16"]["psd_kernel_ms"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,9 +211,20 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB copy (PMC calibration)")
+    ap.add_argument("--e2e", nargs="*", metavar="CONFIG",
+                    help="only: end-to-end CLI search time of %s (default all)" % ",".join(E2E_CONFIGS))
+    ap.add_argument("--fp32-parity", type=int, metavar="ORDER", choices=(45, 50, 90),
+                    help="only: uncompression of a whole level on both PSD paths, identical output and time ratio")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.e2e is not None:
+        print(json.dumps(e2e(args.e2e or list(E2E_CONFIGS))))
+        return
+    if args.fp32_parity:
+        cfg = {45: (45, 3, 1, -2), 50: (50, 5, 1, 0), 90: (90, 10, 5, 0)}[args.fp32_parity]
+        print(json.dumps(fp32_parity(*cfg)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the search runs on a single MI355X (no multi-GPU path is built); "

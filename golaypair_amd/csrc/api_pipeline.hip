@@ -78,23 +78,12 @@ int gp_generate(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int 
             g_gen_stats = s;
             g_gen_key = key;
         }
-        if (stats) {
-            stats->words += g_gen_stats.words;
-            stats->psd_decisions += g_gen_stats.psd_decisions;
-            stats->psd_pass += g_gen_stats.psd_pass;
-            stats->band_rechecks += g_gen_stats.band_rechecks;
-            stats->canonical += g_gen_stats.canonical;
-            stats->tie_band += g_gen_stats.tie_band;
-            stats->kernel_ms += g_gen_stats.kernel_ms;
-            stats->psd_kernel_ms += g_gen_stats.psd_kernel_ms;
-            stats->band_kernel_ms += g_gen_stats.band_kernel_ms;
-            stats->psd_launches += g_gen_stats.psd_launches;
-        }
         int ra = export_cands(g_gen_out.a, out_a);
         int rb = export_cands(g_gen_out.b, out_b);
         if (ra == GP_ERR_CAPACITY || rb == GP_ERR_CAPACITY) return GP_ERR_CAPACITY;
         if (ra != GP_OK) return ra;
         if (rb != GP_OK) return rb;
+        add_stats(stats, g_gen_stats);  // reported once, by the call that returns the candidates
         g_gen_key = GenCacheKey{-1, 0, 0, 0, 0, 0};
         g_gen_out = GenOutput{};
         return GP_OK;

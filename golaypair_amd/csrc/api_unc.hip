@@ -97,8 +97,10 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             if (rc == GP_OK && P > 0 && tmp_b) rc = export_cands(ctx->unc_cache_tb, tmp_b);
             if (rc == GP_OK) rc = export_pairs(ctx->unc_cache_out, out, true);
             if (rc == GP_ERR_CAPACITY) ctx->unc_cache_valid = true;
+            else if (rc == GP_OK) add_stats(stats, ctx->unc_cache_stats);
             return rc;
         }
+        gp_stats local{};  // reported by the call that returns the pairs
         PairSet result;
         result.n = N;
         CandSet last_a, last_b;
@@ -112,13 +114,16 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         }
         // batches by leaf budget; the host tables of batch i+1 are prepared on
         // another thread while the device runs batch i
+        // (the first batch is a quarter of the budget, so that the device starts
+        // while the host is still preparing the second)
         std::vector<std::pair<int64_t, int64_t>> batches;
         for (int64_t q0 = 0; q0 < P;) {
+            const uint64_t budget = batches.empty() ? kLeafBudget / 4 : kLeafBudget;
             uint64_t leaves = 0;
             int64_t q1 = q0;
             while (q1 < P) {
                 const uint64_t add = all[2 * q1].leaves + all[2 * q1 + 1].leaves;
-                if (q1 > q0 && leaves + add > kLeafBudget) break;
+                if (q1 > q0 && leaves + add > budget) break;
                 leaves += add;
                 ++q1;
             }
@@ -186,12 +191,12 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             UncBatch B;
             {
                 Phase ph("uncompress: filter batch");
-                B = unc_launch(ctx, *prep, stats);
+                B = unc_launch(ctx, *prep, &local);
             }
             auto E = std::make_shared<UncEmitted>();
             {
                 Phase ph("uncompress: join batch");
-                *E = unc_join(ctx, B, paf * new_compress, stats);
+                *E = unc_join(ctx, B, paf * new_compress, &local);
             }
             if (q1 == P && (tmp_a || tmp_b)) {  // the last pair's candidate files
                 last_a = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0)));
@@ -204,10 +209,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             });
         }
         if (ordered.valid()) ordered.get();
-        if (stats) {
-            stats->tie_band += ord_ties;
-            stats->pairs += ord_pairs;
-        }
+        local.tie_band += ord_ties;
+        local.pairs += ord_pairs;
         int rc = GP_OK;
         if (P > 0) {
             if (tmp_a) rc = export_cands(last_a, tmp_a);
@@ -228,6 +231,9 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             ctx->unc_cache_ta = std::move(last_a);
             ctx->unc_cache_tb = std::move(last_b);
             ctx->unc_cache_valid = true;
+            ctx->unc_cache_stats = local;
+        } else if (rc == GP_OK) {
+            add_stats(stats, local);
         }
         return rc;
     });

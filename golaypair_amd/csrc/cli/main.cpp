@@ -312,6 +312,8 @@ int main(int argc, char** argv) {
     if (mkdirs(o.work_dir()) != 0) { std::cerr << "Failed to create temp directories\n"; return 1; }
     int rc = gp_create(&g_ctx, 0);
     if (rc != GP_OK) { std::cerr << "golaypair: no usable GPU (gp_create returned " << rc << ")\n"; return 1; }
+    // GOLAY_PSD=fp32 selects the plain FP32 PSD filter (same outputs, slower)
+    if (const char* m = std::getenv("GOLAY_PSD"); m && std::string(m) == "fp32") gp_set_psd_mode(g_ctx, GP_PSD_FP32);
     const std::string wd = o.work_dir();
     const std::string V = std::to_string(o.order);
     auto cand = [&](const std::string& prefix, char ab, int j) { return wd + "/" + V + "-" + prefix + "-" + ab + "#" + std::to_string(j); };

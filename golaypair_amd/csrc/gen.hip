@@ -17,6 +17,7 @@
 #include <cstring>
 #include <numeric>
 #include <set>
+#include <thread>
 
 #include "gp_internal.hpp"
 #include "gp_host.hpp"
@@ -421,19 +422,32 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     std::vector<int64_t> ia, ib;
     for (int64_t i = 0; i < acc_n; ++i) (side_h[i] ? ib : ia).push_back(i);
     if (stats) stats->psd_pass += ia.size() + ib.size();
-    // packed big-endian keys (byte v + 128 per letter): unsigned order = signed lexicographic order
-    const int W = (n + 7) / 8;
-    std::vector<uint64_t> pk((size_t)acc_n * W, 0);
-    for (int64_t i = 0; i < acc_n; ++i)
-        for (int j = 0; j < n; ++j)
-            pk[i * W + j / 8] |= (uint64_t)(uint8_t)(words_h[i * n + j] + 128) << (56 - 8 * (j % 8));
-    auto lexless = [&](int64_t x, int64_t y) {
-        for (int w = 0; w < W; ++w)
-            if (pk[x * W + w] != pk[y * W + w]) return pk[x * W + w] < pk[y * W + w];
-        return false;
+    // packed big-endian keys (byte v + 128 per letter): unsigned order = signed
+    // lexicographic order; (key, row) records sorted directly, A and B concurrently
+    struct Rec {
+        uint64_t k[4];
+        int64_t i;
     };
-    std::sort(ia.begin(), ia.end(), lexless);
-    std::sort(ib.begin(), ib.end(), lexless);
+    const int W = (n + 7) / 8;
+    auto sort_side = [&](std::vector<int64_t>& idx) {
+        std::vector<Rec> rec(idx.size());
+        for (size_t x = 0; x < idx.size(); ++x) {
+            Rec& R = rec[x];
+            R.k[0] = R.k[1] = R.k[2] = R.k[3] = 0;
+            R.i = idx[x];
+            for (int j = 0; j < n; ++j)
+                R.k[j / 8] |= (uint64_t)(uint8_t)(words_h[R.i * n + j] + 128) << (56 - 8 * (j % 8));
+        }
+        std::sort(rec.begin(), rec.end(), [W](const Rec& x, const Rec& y) {
+            for (int w = 0; w < W; ++w)
+                if (x.k[w] != y.k[w]) return x.k[w] < y.k[w];
+            return false;
+        });
+        for (size_t x = 0; x < idx.size(); ++x) idx[x] = rec[x].i;
+    };
+    std::thread ta([&] { sort_side(ia); });
+    sort_side(ib);
+    ta.join();
     auto emit = [&](const std::vector<int64_t>& idx, CandSet& cs) {
         cs.n = n;
         cs.nkey = K;

@@ -99,6 +99,24 @@ struct Phase {
     }
 };
 
+// dst += src, field by field.
+inline void add_stats(gp_stats* dst, const gp_stats& src) {
+    if (!dst) return;
+    dst->words += src.words;
+    dst->psd_decisions += src.psd_decisions;
+    dst->psd_pass += src.psd_pass;
+    dst->band_rechecks += src.band_rechecks;
+    dst->key_rechecks += src.key_rechecks;
+    dst->canonical += src.canonical;
+    dst->pair_checks += src.pair_checks;
+    dst->pairs += src.pairs;
+    dst->tie_band += src.tie_band;
+    dst->kernel_ms += src.kernel_ms;
+    dst->psd_kernel_ms += src.psd_kernel_ms;
+    dst->band_kernel_ms += src.band_kernel_ms;
+    dst->psd_launches += src.psd_launches;
+}
+
 }  // namespace gp
 
 struct gp_ctx {
@@ -116,6 +134,7 @@ struct gp_ctx {
     uint64_t unc_cache_key = 0;
     gp::PairSet unc_cache_out;
     gp::CandSet unc_cache_ta, unc_cache_tb;
+    gp_stats unc_cache_stats{};
     // named scratch buffers of the pipeline stages
     std::map<std::string, std::unique_ptr<gp::DevBuf>> pool;
     gp::DevBuf& buf(const std::string& name) {
