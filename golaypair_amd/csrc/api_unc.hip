@@ -139,10 +139,15 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             const auto [q0, q1] = batches[i];
             const std::span<const SideHost> sides(all.data() + 2 * q0, (size_t)(2 * (q1 - q0)));
             const std::span<const int> grp(group.data(), sides.size());
-            return unc_prepare(sides, grp, order, paf, fp32, trig);
+            auto prep = unc_prepare(sides, grp, order, paf, fp32, trig);
+            GP_HIP(hipSetDevice(ctx->device));
+            unc_stage(*prep, ctx->pinned);
+            return prep;
         };
-        std::future<std::shared_ptr<UncPrep>> next;
+        // the first two batches are prepared together; batch i + 2 while batch i runs
+        std::future<std::shared_ptr<UncPrep>> next, after;
         if (!batches.empty()) next = std::async(std::launch::async, prepare, 0);
+        if (batches.size() > 1) after = std::async(std::launch::async, prepare, 1);
         // Emitted pairs of one batch in the reference's order (input pair, then A
         // line text, then B line text), appended to `result`.  Runs on a host
         // thread while the device works on the next batch; tasks are chained so
@@ -187,7 +192,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         for (size_t bi = 0; bi < batches.size(); ++bi) {
             const auto [q0, q1] = batches[bi];
             std::shared_ptr<UncPrep> prep = next.get();
-            if (bi + 1 < batches.size()) next = std::async(std::launch::async, prepare, bi + 1);
+            next = std::move(after);
+            if (bi + 2 < batches.size()) after = std::async(std::launch::async, prepare, bi + 2);
             UncBatch B;
             {
                 Phase ph("uncompress: filter batch");

@@ -24,6 +24,7 @@
 
 #include "gp_host.hpp"
 #include "gp_internal.hpp"
+#include "linesort.hpp"
 
 namespace gp {
 
@@ -195,6 +196,7 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     if (m == 0) return GP_OK;
     if (in.n != n) throw ArgError("filter: pair length != order/compress");
     if (n > kMaxN) throw ArgError("filter: length above 64");
+    Phase ph("filter: packing setup");
     FilterParams P{};
     P.n = n;
     P.alt = !(compress > 1 || n % 2 == 1);  // filter.cpp:83-87
@@ -202,14 +204,15 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
         if (std::gcd(u, n) == 1) P.units[P.nunits++] = u;
     if (n == 1) { P.nunits = 1; P.units[0] = 0; }
     // key packing: the values and their negations lie in [-mx, mx]; halve when all have mx's parity
-    int mx = 0;
-    bool same_parity = true;
+    int mx = 0, odd_or = 0, odd_and = 1;  // max |v|; whether some / every value is odd
     for (int64_t i = 0; i < m * n; ++i) {
-        mx = std::max(mx, std::abs((int)in.a[i]));
-        mx = std::max(mx, std::abs((int)in.b[i]));
+        const int a = in.a[i], b = in.b[i];
+        mx = std::max(mx, std::max(a < 0 ? -a : a, b < 0 ? -b : b));
+        odd_or |= (a | b) & 1;
+        odd_and &= a & b & 1;
     }
-    for (int64_t i = 0; i < m * n && same_parity; ++i)
-        same_parity = ((in.a[i] + mx) % 2 == 0) && ((in.b[i] + mx) % 2 == 0);
+    // every v + mx even: all values odd with mx odd, or all even with mx even
+    const bool same_parity = (mx & 1) ? odd_and == 1 : odd_or == 0;
     P.mx = mx;
     P.sh = same_parity ? 1 : 0;
     const int range = (2 * mx) >> P.sh;
@@ -218,6 +221,7 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     const int wneed = (n + P.E - 1) / P.E;
     const int W = wneed <= 1 ? 1 : wneed <= 2 ? 2 : wneed <= 4 ? 4 : 8;
     hipStream_t st = ctx->stream;
+    ph.next("filter: device");
     int8_t* da = ctx->dev<int8_t>("f_a", (size_t)m * n);
     int8_t* db = ctx->dev<int8_t>("f_b", (size_t)m * n);
     uint64_t* dc = ctx->dev<uint64_t>("f_c", (size_t)m * 2 * W);
@@ -237,32 +241,43 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     float ms = 0;
     GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (stats) stats->kernel_ms += ms;
-    // lexicographic order on (a, b) as signed ints (golay.h:25-31)
+    ph.next("filter: group + order");
+    // Pairs with equal canonical keys form one class; its survivor is the pair
+    // that comes first in (a, b) lexicographic order (golay.h:25-31).  Both orders
+    // use packed big-endian keys (byte v + 128 per entry, a then b).
+    const int PW = (2 * n + 7) / 8;
+    std::vector<uint64_t> pk((size_t)m * PW, 0);
+    for (int64_t i = 0; i < m; ++i)
+        for (int j = 0; j < 2 * n; ++j) {
+            const int v = j < n ? in.a[i * n + j] : in.b[i * n + j - n];
+            pk[i * PW + j / 8] |= (uint64_t)(uint8_t)(v + 128) << (56 - 8 * (j % 8));
+        }
     auto pair_less = [&](int64_t x, int64_t y) {
-        for (int i = 0; i < n; ++i)
-            if (in.a[x * n + i] != in.a[y * n + i]) return in.a[x * n + i] < in.a[y * n + i];
-        for (int i = 0; i < n; ++i)
-            if (in.b[x * n + i] != in.b[y * n + i]) return in.b[x * n + i] < in.b[y * n + i];
+        for (int w = 0; w < PW; ++w)
+            if (pk[x * PW + w] != pk[y * PW + w]) return pk[x * PW + w] < pk[y * PW + w];
         return false;
     };
-    auto canon_cmp = [&](int64_t x, int64_t y) {
+    auto canon_less = [&](int64_t x, int64_t y) {
         for (int w = 0; w < 2 * W; ++w)
-            if (canon[x * 2 * W + w] != canon[y * 2 * W + w]) return canon[x * 2 * W + w] < canon[y * 2 * W + w] ? -1 : 1;
-        return 0;
+            if (canon[x * 2 * W + w] != canon[y * 2 * W + w]) return canon[x * 2 * W + w] < canon[y * 2 * W + w];
+        return false;
     };
     std::vector<int64_t> idx(m);
     std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
-        int c = canon_cmp(x, y);
-        return c != 0 ? c < 0 : pair_less(x, y);
+    parallel_sort(idx, [&](int64_t x, int64_t y) {
+        if (canon_less(x, y)) return true;
+        if (canon_less(y, x)) return false;
+        return pair_less(x, y);
     });
     std::vector<int64_t> keep;
     for (int64_t r = 0; r < m; ++r)
-        if (r == 0 || canon_cmp(idx[r], idx[r - 1]) != 0) keep.push_back(idx[r]);
+        if (r == 0 || canon_less(idx[r - 1], idx[r])) keep.push_back(idx[r]);
     std::sort(keep.begin(), keep.end(), pair_less);
-    for (int64_t i : keep) {
-        out.a.insert(out.a.end(), &in.a[i * n], &in.a[i * n] + n);
-        out.b.insert(out.b.end(), &in.b[i * n], &in.b[i * n] + n);
+    out.a.resize(keep.size() * n);
+    out.b.resize(keep.size() * n);
+    for (size_t r = 0; r < keep.size(); ++r) {
+        std::memcpy(&out.a[r * n], &in.a[keep[r] * n], n);
+        std::memcpy(&out.b[r * n], &in.b[keep[r] * n], n);
     }
     if (stats) stats->pairs += keep.size();
     return GP_OK;

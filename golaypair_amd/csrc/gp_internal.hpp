@@ -10,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -117,6 +118,48 @@ inline void add_stats(gp_stats* dst, const gp_stats& src) {
     dst->psd_launches += src.psd_launches;
 }
 
+// Pinned host buffers for staging uploads, shared by host threads.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+};
+struct PinnedPool {
+    std::mutex mu;
+    std::vector<std::unique_ptr<PinnedBuf>> bufs;
+    PinnedBuf* acquire(size_t n) {
+        std::lock_guard<std::mutex> lk(mu);
+        PinnedBuf* pick = nullptr;
+        for (auto& b : bufs)
+            if (!b->busy && b->bytes >= n) { pick = b.get(); break; }
+        if (!pick)
+            for (auto& b : bufs)
+                if (!b->busy) { pick = b.get(); break; }
+        if (!pick) {
+            bufs.emplace_back(new PinnedBuf());
+            pick = bufs.back().get();
+        }
+        if (pick->bytes < n) {
+            if (pick->p) GP_HIP(hipHostFree(pick->p));
+            pick->p = nullptr;
+            pick->bytes = 0;
+            const size_t want = n + n / 4 + 4096;
+            GP_HIP(hipHostMalloc(&pick->p, want, hipHostMallocDefault));
+            pick->bytes = want;
+        }
+        pick->busy = true;
+        return pick;
+    }
+    void release(PinnedBuf* b) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (b) b->busy = false;
+    }
+    ~PinnedPool() {
+        for (auto& b : bufs)
+            if (b->p) (void)hipHostFree(b->p);
+    }
+};
+
 }  // namespace gp
 
 struct gp_ctx {
@@ -135,6 +178,7 @@ struct gp_ctx {
     gp::PairSet unc_cache_out;
     gp::CandSet unc_cache_ta, unc_cache_tb;
     gp_stats unc_cache_stats{};
+    gp::PinnedPool pinned;  // staging for batch uploads
     // named scratch buffers of the pipeline stages
     std::map<std::string, std::unique_ptr<gp::DevBuf>> pool;
     gp::DevBuf& buf(const std::string& name) {

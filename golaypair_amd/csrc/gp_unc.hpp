@@ -83,6 +83,10 @@ struct UncPrep;
 std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
                                      bool fp32, const UncTrig& trig);
 UncBatch unc_launch(gp_ctx* ctx, UncPrep& prep, gp_stats* stats);
+// Copies a prepared batch's tables into one pinned staging buffer from `pool`
+// (host memory only; unc_launch stages a batch that was not staged).
+struct PinnedPool;
+void unc_stage(UncPrep& prep, PinnedPool& pool);
 
 // group[i]: pair index of input side i (a-side seqflag 0, b-side seqflag 1).
 UncBatch unc_filter(gp_ctx* ctx, std::span<const SideHost> sides, std::span<const int> group, int order, int paf,

@@ -54,6 +54,29 @@ void build_lines(int64_t m, F append, std::string& text, std::vector<size_t>& of
     off.push_back(text.size());
 }
 
+// Sorts idx with `less` on host threads: chunks sorted concurrently, then merged pairwise.
+template <class Less>
+void parallel_sort(std::vector<int64_t>& idx, const Less& less, int threads = 16) {
+    const int64_t m = (int64_t)idx.size();
+    const int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
+    std::vector<int64_t> cut(nt + 1);
+    for (int t = 0; t <= nt; ++t) cut[t] = m * t / nt;
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
+        for (auto& x : th) x.join();
+    }
+    for (int w = 1; w < nt; w *= 2) {
+        std::vector<std::thread> th;
+        for (int t = 0; t + w < nt; t += 2 * w) {
+            const int64_t a = cut[t], b = cut[t + w], e = cut[std::min(t + 2 * w, nt)];
+            th.emplace_back([&, a, b, e] { std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less); });
+        }
+        for (auto& x : th) x.join();
+    }
+}
+
 inline void line_keys(const LineSet& L, int64_t m, std::vector<uint64_t>& hi, std::vector<uint64_t>& lo) {
     hi.assign(m, 0);
     lo.assign(m, 0);
@@ -83,23 +106,7 @@ inline void sort_lines(const LineSet& L, std::vector<int64_t>& idx, int threads 
         const size_t k = std::min(L.skip + 16, std::min(a.size(), b.size()));
         return a.substr(k) < b.substr(k);
     };
-    int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
-    std::vector<int64_t> cut(nt + 1);
-    for (int t = 0; t <= nt; ++t) cut[t] = m * t / nt;
-    {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&, t] { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
-        for (auto& x : th) x.join();
-    }
-    for (int w = 1; w < nt; w *= 2) {
-        std::vector<std::thread> th;
-        for (int t = 0; t + w < nt; t += 2 * w) {
-            const int64_t a = cut[t], b = cut[t + w], e = cut[std::min(t + 2 * w, nt)];
-            th.emplace_back([&, a, b, e] { std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less); });
-        }
-        for (auto& x : th) x.join();
-    }
+    parallel_sort(idx, less, threads);
 }
 
 }  // namespace gp

@@ -697,7 +697,56 @@ struct UncPrep {
     std::vector<UncClass> classes;
     uint64_t total_leaves = 0;
     uint64_t dead_leaves = 0;  // leaves of sides rejected by their fixed channels
+    // every table in one pinned staging buffer (one host-to-device copy)
+    PinnedBuf* stage = nullptr;
+    PinnedPool* stage_pool = nullptr;
+    size_t stage_bytes = 0;
+    std::vector<size_t> stage_off;
+    ~UncPrep() {
+        if (stage_pool) stage_pool->release(stage);
+    }
 };
+
+namespace {
+
+struct HostSpan {
+    const void* p;
+    size_t bytes;
+};
+
+template <class V>
+HostSpan span_of(const std::vector<V>& v) {
+    return HostSpan{v.data(), v.size() * sizeof(V)};
+}
+
+// The tables of a batch in staging order (the order unc_launch reads them back).
+std::vector<HostSpan> stage_spans(const UncPrep& U, const int64_t* toff) {
+    const Tables& T = U.T;
+    const Shared& SH = U.SH;
+    return {span_of(T.sides), span_of(T.radix), span_of(T.tup),   span_of(T.pm1),   span_of(SH.c64),
+            span_of(T.pos64), span_of(T.pos32), span_of(T.olist), span_of(T.fix32), span_of(SH.c32),
+            span_of(SH.mid),  span_of(SH.sufb), span_of(SH.frags), span_of(SH.ent),  span_of(SH.w32),
+            HostSpan{toff, sizeof(int64_t)}};
+}
+
+}  // namespace
+
+void unc_stage(UncPrep& U, PinnedPool& pool) {
+    if (U.stage || U.T.sides.empty()) return;
+    const int64_t toff = U.trig_off;
+    const std::vector<HostSpan> sp = stage_spans(U, &toff);
+    U.stage_off.assign(sp.size(), 0);
+    size_t total = 0;
+    for (size_t i = 0; i < sp.size(); ++i) {
+        U.stage_off[i] = total;
+        total += (sp[i].bytes + 255) & ~(size_t)255;
+    }
+    U.stage = pool.acquire(total);
+    U.stage_pool = &pool;
+    U.stage_bytes = total;
+    for (size_t i = 0; i < sp.size(); ++i)
+        if (sp[i].bytes) std::memcpy(static_cast<char*>(U.stage->p) + U.stage_off[i], sp[i].p, sp[i].bytes);
+}
 
 template <int NBS, int MT, bool HALF>
 void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl, uint64_t* surv,
@@ -942,23 +991,26 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     }
     if (T.sides.empty()) return B;
     Phase ph2("unc_filter: upload");
-    B.d_sides = upload(ctx, "u_sides", T.sides);
-    B.d_radix = upload(ctx, "u_radix", T.radix);
-    B.d_tup = upload(ctx, "u_tup", T.tup);
-    B.d_pm1 = upload(ctx, "u_pm1", T.pm1);
-    B.d_c64 = upload(ctx, "u_c64", SH.c64);
-    B.d_pos64 = upload(ctx, "u_pos64", T.pos64);
-    auto* d_pos32 = upload(ctx, "u_pos32", T.pos32);
-    auto* d_olist = upload(ctx, "u_olist", T.olist);
-    auto* d_fix32 = upload(ctx, "u_fix32", T.fix32);
-    auto* d_c32 = upload(ctx, "u_c32", SH.c32);
-    auto* d_mid = upload(ctx, "u_mid", SH.mid);
-    auto* d_sufb = upload(ctx, "u_sufb", SH.sufb);
-    auto* d_frags = upload(ctx, "u_frags", SH.frags);
-    auto* d_ent = upload(ctx, "u_ent", SH.ent);
-    auto* d_w32 = upload(ctx, "u_w32", SH.w32);
-    std::vector<int64_t> toff{U.trig_off};
-    auto* d_toff = upload(ctx, "u_toff", toff);
+    unc_stage(U, ctx->pinned);
+    char* base = static_cast<char*>(ctx->buf("u_stage").ensure(U.stage_bytes));
+    GP_HIP(hipMemcpyAsync(base, U.stage->p, U.stage_bytes, hipMemcpyHostToDevice, st));
+    auto at = [&](int i) { return static_cast<void*>(base + U.stage_off[i]); };
+    B.d_sides = static_cast<const UncSide*>(at(0));
+    B.d_radix = static_cast<const int32_t*>(at(1));
+    B.d_tup = static_cast<const int8_t*>(at(2));
+    B.d_pm1 = static_cast<const uint64_t*>(at(3));
+    B.d_c64 = static_cast<const double*>(at(4));
+    B.d_pos64 = static_cast<const int32_t*>(at(5));
+    auto* d_pos32 = static_cast<const int32_t*>(at(6));
+    auto* d_olist = static_cast<const int32_t*>(at(7));
+    auto* d_fix32 = static_cast<const float*>(at(8));
+    auto* d_c32 = static_cast<const float*>(at(9));
+    auto* d_mid = static_cast<const float*>(at(10));
+    auto* d_sufb = static_cast<const uint4*>(at(11));
+    auto* d_frags = static_cast<const uint4*>(at(12));
+    auto* d_ent = static_cast<const int8_t*>(at(13));
+    auto* d_w32 = static_cast<const float*>(at(14));
+    auto* d_toff = static_cast<const int64_t*>(at(15));
     B.nsides_table = (int)T.sides.size();
     UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
     int64_t scap = (int64_t)std::min<uint64_t>(total_leaves * 3 / 5 + (1 << 20), (uint64_t)3 << 30);
