@@ -93,6 +93,7 @@ int gp_generate(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int 
 int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
     if (!ctx) return GP_ERR_ARG;
     return guarded_call(ctx, [&] {
+        Phase ph("sort: import");
         CandSet s = import_cands(c);
         const int64_t m = s.size();
         if (m == 0) return GP_OK;
@@ -100,12 +101,15 @@ int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
         if (K != n / 2) throw ArgError("gp_sort_candidates: nkey must be n/2");
         std::string text;
         std::vector<size_t> off;
+        ph.next("sort: lines");
         build_lines(m, [&](std::string& t, int64_t i) { append_cand_line(t, &s.words[i * n], &s.keys[i * K], n); },
                     text, off);
         std::vector<int64_t> idx(m);
         std::iota(idx.begin(), idx.end(), 0);
         auto line = [&](int64_t i) { return std::string_view(text.data() + off[i], off[i + 1] - off[i]); };
+        ph.next("sort: sort");
         sort_lines(LineSet{&text, &off, 3}, idx);  // every line starts with "PSD"
+        ph.next("sort: uniq + copy");
         int64_t w = 0;
         for (int64_t r = 0; r < m; ++r) {
             if (w > 0 && line(idx[r]) == line(idx[w - 1])) continue;  // uniq

@@ -12,6 +12,7 @@
 #include "gp_unc.hpp"
 #include "linesort.hpp"
 #include "textfmt.hpp"
+#include "threadpool.hpp"
 
 using namespace gp;
 
@@ -28,23 +29,10 @@ void build_sides(const PairSet& inp, int L, int c, int c2, int64_t q0, int64_t q
     const int64_t n = q1 - q0;
     out.resize(2 * n);
     const auto tab = make_splits(c, c2);
-    const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
-    std::vector<std::thread> th;
-    std::vector<std::string> err(nt);
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            try {
-                for (int64_t i = t; i < n; i += nt) {
-                    out[2 * i] = build_side(tab, &inp.a[(q0 + i) * L], L, 0);
-                    out[2 * i + 1] = build_side(tab, &inp.b[(q0 + i) * L], L, 1);
-                }
-            } catch (const std::exception& e) {
-                err[t] = e.what();
-            }
-        });
-    for (auto& x : th) x.join();
-    for (auto& e : err)
-        if (!e.empty()) throw ArgError(e);
+    pool().parallel_for((size_t)n, [&](size_t i) {
+        out[2 * i] = build_side(tab, &inp.a[(q0 + i) * L], L, 0);
+        out[2 * i + 1] = build_side(tab, &inp.b[(q0 + i) * L], L, 1);
+    });
 }
 
 }  // namespace

@@ -18,6 +18,7 @@
 #include <numeric>
 #include <set>
 #include <thread>
+#include <type_traits>
 
 #include "gp_internal.hpp"
 #include "gp_host.hpp"
@@ -53,29 +54,40 @@ struct GenCtl {
     unsigned long long words;
 };
 
-__device__ __forceinline__ int digit_at(u128 w, int n, int p) { return (int)((w >> (4 * (n - 1 - p))) & 15); }
+// Words are packed 4 bits per letter index, first letter most significant:
+// uint64_t for n <= 16, u128 up to n = 32.  Lexicographic order = unsigned order.
+template <class Wd>
+__device__ __forceinline__ int digit_at(Wd w, int n, int p) { return (int)((w >> (4 * (n - 1 - p))) & 15); }
 
-__device__ __forceinline__ u128 mask_n(int n) {
-    return n >= 32 ? ~(u128)0 : (((u128)1 << (4 * n)) - 1);
+template <class Wd>
+__device__ __forceinline__ Wd mask_n(int n) {
+    return 4 * n >= (int)(8 * sizeof(Wd)) ? ~(Wd)0 : (((Wd)1 << (4 * n)) - 1);
 }
 
-__device__ __forceinline__ u128 rot(u128 w, int n, int t, u128 mask) {
+template <class Wd>
+__device__ __forceinline__ Wd rot(Wd w, int n, int t, Wd mask) {
     if (t == 0) return w;
     return ((w << (4 * t)) | (w >> (4 * (n - t)))) & mask;
 }
 
-// w <= every rotation of v (v may equal w).
-__device__ __forceinline__ bool le_all_rotations(u128 w, u128 v, int n, u128 mask, int t0) {
-    for (int t = t0; t < n; ++t)
+// w <= every rotation of v (v may equal w).  A rotation can only be smaller if
+// it starts with a letter <= w's first letter, so only those are compared.
+template <class Wd>
+__device__ __forceinline__ bool le_all_rotations(Wd w, Wd v, int n, Wd mask, int t0) {
+    const int w0 = digit_at(w, n, 0);
+    for (int t = t0; t < n; ++t) {
+        if (digit_at(v, n, t) > w0) continue;
         if (rot(v, n, t, mask) < w) return false;
+    }
     return true;
 }
 
-__device__ __forceinline__ u128 decimate(u128 w, int n, int u) {
-    u128 r = 0;
+template <class Wd>
+__device__ __forceinline__ Wd decimate(Wd w, int n, int u) {
+    Wd r = 0;
     int idx = 0;
     for (int i = 0; i < n; ++i) {
-        r = (r << 4) | (u128)digit_at(w, n, idx);
+        r = (r << 4) | (Wd)digit_at(w, n, idx);
         idx += u;
         if (idx >= n) idx -= n;
     }
@@ -83,8 +95,9 @@ __device__ __forceinline__ u128 decimate(u128 w, int n, int u) {
 }
 
 // Unrank rank r of the words with index sum J (lexicographic order).
-__device__ u128 unrank(int64_t r, int J, int n, int k, const uint64_t* __restrict__ cnt, int cstride) {
-    u128 w = 0;
+template <class Wd>
+__device__ Wd unrank(int64_t r, int J, int n, int k, const uint64_t* __restrict__ cnt, int cstride) {
+    Wd w = 0;
     for (int p = 0; p < n; ++p) {
         const int rem = n - 1 - p;
         int d = 0;
@@ -94,27 +107,28 @@ __device__ u128 unrank(int64_t r, int J, int n, int k, const uint64_t* __restric
             if (r < c) break;
             r -= c;
         }
-        w = (w << 4) | (u128)d;
+        w = (w << 4) | (Wd)d;
         J -= d;
     }
     return w;
 }
 
 // Next word with the same digit sum in lexicographic order (false at the end).
-__device__ bool successor(u128& w, int n, int k) {
+template <class Wd>
+__device__ bool successor(Wd& w, int n, int k) {
     int S = 0;
     for (int p = n - 2; p >= 0; --p) {
         S += digit_at(w, n, p + 1);
         const int d = digit_at(w, n, p);
         if (d < k - 1 && S >= 1) {
             // digits 0..p-1 unchanged, digit p + 1, suffix = smallest arrangement of S - 1
-            u128 head = w >> (4 * (n - p));          // digits 0..p-1
-            u128 out = (head << 4) | (u128)(d + 1);
-            int rem = S - 1;
+            const Wd head = p > 0 ? w >> (4 * (n - p)) : (Wd)0;  // digits 0..p-1
+            const Wd out = (head << 4) | (Wd)(d + 1);
+            const int rem = S - 1;
             const int L = n - 1 - p;
-            u128 suf = 0;
-            int full = rem / (k - 1);
-            int part = rem - full * (k - 1);
+            Wd suf = 0;
+            const int full = rem / (k - 1);
+            const int part = rem - full * (k - 1);
             // suffix (left to right): zeros, then `part` (if full < L), then full x (k-1)
             for (int q = 0; q < L; ++q) {
                 int v;
@@ -122,7 +136,7 @@ __device__ bool successor(u128& w, int n, int k) {
                 if (from_right < full) v = k - 1;
                 else if (from_right == full) v = part;
                 else v = 0;
-                suf = (suf << 4) | (u128)v;
+                suf = (suf << 4) | (Wd)v;
             }
             w = (out << (4 * L)) | suf;
             return true;
@@ -131,6 +145,7 @@ __device__ bool successor(u128& w, int n, int k) {
     return false;
 }
 
+template <class Wd>
 __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __restrict__ segs,
                                                   const GenTask* __restrict__ tasks, int64_t ntasks,
                                                   const uint64_t* __restrict__ cnt, int cstride,
@@ -141,16 +156,16 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int n = P.n, k = P.k;
-    const u128 mask = mask_n(n);
+    const Wd mask = mask_n<Wd>(n);
     unsigned long long nwords = 0;
     for (int64_t ti = wave; ti < ntasks; ti += nwaves) {
         const GenTask T = tasks[ti];
         const GenSeg S = segs[T.seg];
         int64_t r0 = T.start + (int64_t)lane * T.per_lane;
         int64_t r1 = min(r0 + (int64_t)T.per_lane, T.end);
-        u128 w = 0;
+        Wd w = 0;
         bool live = r0 < r1;
-        if (live) w = unrank(r0, S.J, n, k, cnt, cstride);
+        if (live) w = unrank<Wd>(r0, S.J, n, k, cnt, cstride);
         int64_t r = r0;
         // every lane steps until the whole wave is done
         while (__any(live)) {
@@ -190,11 +205,11 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
                 const unsigned long long below = (1ull << lane) - 1ull;
                 if (keepA) {
                     const unsigned long long i = base + __popcll(ba & below);
-                    if ((int64_t)i < P.capacity) { surv_w[i] = w; surv_side[i] = 0; }
+                    if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 0; }
                 }
                 if (keepB) {
                     const unsigned long long i = base + __popcll(ba) + __popcll(bb & below);
-                    if ((int64_t)i < P.capacity) { surv_w[i] = w; surv_side[i] = 1; }
+                    if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 1; }
                 }
             }
             if (live) {
@@ -349,11 +364,17 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         GP_HIP(hipMemsetAsync(d_ctl.ensure(sizeof(GenCtl)), 0, sizeof(GenCtl), st));
         GP_HIP(hipEventRecord(ctx->ev0, st));
         const int64_t waves = std::min<int64_t>((int64_t)tasks.size(), (int64_t)ctx->num_cus * 32);
-        if (!tasks.empty())
-            hipLaunchKernelGGL(k_gen_enum, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0, st, P,
-                               d_segs.as<GenSeg>(), d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(),
-                               cstride, bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side,
-                               d_ctl.as<GenCtl>());
+        if (!tasks.empty()) {
+            const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+            if (n <= 16)
+                hipLaunchKernelGGL(k_gen_enum<uint64_t>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
+                                   d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
+                                   bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side, d_ctl.as<GenCtl>());
+            else
+                hipLaunchKernelGGL(k_gen_enum<u128>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
+                                   d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
+                                   bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side, d_ctl.as<GenCtl>());
+        }
         GP_HIP(hipGetLastError());
         GP_HIP(hipEventRecord(ctx->ev1, st));
         GP_HIP(hipMemcpyAsync(&h, d_ctl.p, sizeof h, hipMemcpyDeviceToHost, st));
@@ -424,26 +445,32 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     if (stats) stats->psd_pass += ia.size() + ib.size();
     // packed big-endian keys (byte v + 128 per letter): unsigned order = signed
     // lexicographic order; (key, row) records sorted directly, A and B concurrently
-    struct Rec {
-        uint64_t k[4];
-        int64_t i;
-    };
     const int W = (n + 7) / 8;
     auto sort_side = [&](std::vector<int64_t>& idx) {
-        std::vector<Rec> rec(idx.size());
-        for (size_t x = 0; x < idx.size(); ++x) {
-            Rec& R = rec[x];
-            R.k[0] = R.k[1] = R.k[2] = R.k[3] = 0;
-            R.i = idx[x];
-            for (int j = 0; j < n; ++j)
-                R.k[j / 8] |= (uint64_t)(uint8_t)(words_h[R.i * n + j] + 128) << (56 - 8 * (j % 8));
-        }
-        std::sort(rec.begin(), rec.end(), [W](const Rec& x, const Rec& y) {
-            for (int w = 0; w < W; ++w)
-                if (x.k[w] != y.k[w]) return x.k[w] < y.k[w];
-            return false;
-        });
-        for (size_t x = 0; x < idx.size(); ++x) idx[x] = rec[x].i;
+        auto run = [&](auto tag) {
+            constexpr int KW = decltype(tag)::value;
+            struct Rec {
+                uint64_t k[KW];
+                int64_t i;
+            };
+            std::vector<Rec> rec(idx.size());
+            for (size_t x = 0; x < idx.size(); ++x) {
+                Rec& R = rec[x];
+                for (int w = 0; w < KW; ++w) R.k[w] = 0;
+                R.i = idx[x];
+                for (int j = 0; j < n; ++j)
+                    R.k[j / 8] |= (uint64_t)(uint8_t)(words_h[R.i * n + j] + 128) << (56 - 8 * (j % 8));
+            }
+            std::sort(rec.begin(), rec.end(), [](const Rec& x, const Rec& y) {
+                for (int w = 0; w < KW; ++w)
+                    if (x.k[w] != y.k[w]) return x.k[w] < y.k[w];
+                return false;
+            });
+            for (size_t x = 0; x < idx.size(); ++x) idx[x] = rec[x].i;
+        };
+        if (W == 1) run(std::integral_constant<int, 1>{});
+        else if (W == 2) run(std::integral_constant<int, 2>{});
+        else run(std::integral_constant<int, 4>{});
     };
     std::thread ta([&] { sort_side(ia); });
     sort_side(ib);

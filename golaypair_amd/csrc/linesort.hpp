@@ -13,6 +13,8 @@
 #include <thread>
 #include <vector>
 
+#include "threadpool.hpp"
+
 namespace gp {
 
 struct LineSet {
@@ -30,16 +32,13 @@ void build_lines(int64_t m, F append, std::string& text, std::vector<size_t>& of
     const int nt = std::max(1, std::min<int>(threads, (int)(m / 2048)));
     std::vector<std::string> part(nt);
     std::vector<std::vector<size_t>> poff(nt);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            const int64_t a = m * t / nt, b = m * (t + 1) / nt;
-            for (int64_t i = a; i < b; ++i) {
-                poff[t].push_back(part[t].size());
-                append(part[t], i);
-            }
-        });
-    for (auto& x : th) x.join();
+    pool().parallel_for(nt, [&](size_t t) {
+        const int64_t a = m * (int64_t)t / nt, b = m * (int64_t)(t + 1) / nt;
+        for (int64_t i = a; i < b; ++i) {
+            poff[t].push_back(part[t].size());
+            append(part[t], i);
+        }
+    });
     size_t total = 0;
     for (auto& p : part) total += p.size();
     text.clear();
@@ -61,19 +60,14 @@ void parallel_sort(std::vector<int64_t>& idx, const Less& less, int threads = 16
     const int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
     std::vector<int64_t> cut(nt + 1);
     for (int t = 0; t <= nt; ++t) cut[t] = m * t / nt;
-    {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&, t] { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
-        for (auto& x : th) x.join();
-    }
+    pool().parallel_for(nt, [&](size_t t) { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
     for (int w = 1; w < nt; w *= 2) {
-        std::vector<std::thread> th;
-        for (int t = 0; t + w < nt; t += 2 * w) {
+        const int pairs = (nt - w + 2 * w - 1) / (2 * w);
+        pool().parallel_for(pairs, [&](size_t q) {
+            const int t = (int)q * 2 * w;
             const int64_t a = cut[t], b = cut[t + w], e = cut[std::min(t + 2 * w, nt)];
-            th.emplace_back([&, a, b, e] { std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less); });
-        }
-        for (auto& x : th) x.join();
+            std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less);
+        });
     }
 }
 

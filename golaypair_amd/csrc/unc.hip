@@ -22,6 +22,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "threadpool.hpp"
 #include "unc_kernels.inc"  // opens namespace gp (closed at the end of this file)
 
 // ---------------------------------------------------------------------------
@@ -843,15 +844,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 if (sides[i].leaves != 0) live.push_back(i);
         for (int k = 1; k <= K; ++k)
             if (all_channels || k % r != 0) SH.chan.push_back(k);
-        const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
-        auto parallel = [nt](size_t count, const std::function<void(size_t)>& f) {
-            std::vector<std::thread> th;
-            for (int t = 0; t < nt; ++t)
-                th.emplace_back([&, t] {
-                    for (size_t q = t; q < count; q += nt) f(q);
-                });
-            for (auto& x : th) x.join();
-        };
+        auto parallel = [](size_t count, const std::function<void(size_t)>& f) { pool().parallel_for(count, f); };
         // shared per-(position, letter) blocks; per side its split and table keys
         Phase ph1("tables: blocks + keys");
         if (!live.empty()) add_blocks(SH, *sides[live[0]].tab, sides[live[0]].L, N, cs, sn);
