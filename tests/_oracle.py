@@ -138,7 +138,10 @@ def uncompress_side(orig, c, c2, paf, seqflag):
     lib = load()
     o, po = i32(orig)
     leaves = ctypes.c_uint64()
-    n = lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, None, 0, ctypes.byref(leaves))
-    buf = ctypes.create_string_buffer(n + 1)
-    lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, buf, n + 1, ctypes.byref(leaves))
+    cap = 1 << 24  # one pass when the text fits; the call reports the size it needs
+    buf = ctypes.create_string_buffer(cap)
+    n = lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, buf, cap, ctypes.byref(leaves))
+    if n + 1 > cap:
+        buf = ctypes.create_string_buffer(n + 1)
+        lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, buf, n + 1, ctypes.byref(leaves))
     return buf.value.decode(), leaves.value

@@ -89,20 +89,24 @@ def test_uncompress_compress1_is_noop(gpu_ctx):
     assert len(oa) == 0
 
 
-@pytest.mark.parametrize("order,compress,paf,npairs", [(90, [10, 5], 0, 3), (45, [3, 1], -2, 2)])
-def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs):
+@pytest.mark.parametrize("order,compress,paf,npairs,max_leaves,with_filter", [
+    (90, [10, 5], 0, 3, None, True), (45, [3, 1], -2, 2, None, True)])
+def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs, max_leaves, with_filter):
     """Matched pairs of a full-size level (v=90 c=10 -> 5, filtered at length 18;
-    v=45 c=3 -> 1, length 45): GPU gen/sort/match, the first `npairs` input
-    pairs that emit pairs at the finer level (found one pair at a time on the
-    GPU) plus the pair before the first of them, then uncompress and the
-    equivalence filter on the GPU and in the oracle over that slice."""
+    v=45 c=3 -> 1, length 45; v=50 c=5 -> 1, length 50): GPU gen/sort/match, the
+    first `npairs` input pairs that emit pairs at the finer level (found one pair
+    at a time on the GPU, at most `max_leaves` leaves each) plus an input pair
+    that emits nothing, then uncompress (and the equivalence filter) on the GPU
+    and in the oracle over that slice."""
     c, c2 = compress
     a, b, _ = gpu_ctx.generate(order, c, paf)
     rc, pa, pb, _ = gpu_ctx.match(order, c, paf, gpu_ctx.sort_candidates(a), gpu_ctx.sort_candidates(b))
     assert rc == 0
     pick = []
-    for i in range(min(len(pa), 400)):
-        oa, _, _, _, _ = gpu_ctx.uncompress(order, c, c2, paf, pa[i:i + 1], pb[i:i + 1])
+    for i in range(min(len(pa), 400 if max_leaves is None else len(pa))):
+        oa, _, _, _, st = gpu_ctx.uncompress(order, c, c2, paf, pa[i:i + 1], pb[i:i + 1])
+        if max_leaves is not None and st.psd_decisions > max_leaves:
+            continue  # keep the oracle's share of the test to seconds
         if len(oa):
             if not pick and i > 0:
                 pick.append(i - 1)  # an input pair that emits nothing
@@ -116,12 +120,34 @@ def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs):
     os.makedirs(wd)
     open(os.path.join(wd, "%d-pairs-%d" % (order, c)), "w").write(
         "".join(g.pair_line(x, y) + "\n" for x, y in zip(pa, pb)))
-    subprocess.run([_oracle.CLI, str(order), "-c", str(c), str(c2), "--paf", str(paf), "-d", d, "uncompress",
-                    "filter"], check=True, capture_output=True, timeout=600)
+    subprocess.run([_oracle.CLI, str(order), "-c", str(c), str(c2), "--paf", str(paf), "-d", d, "uncompress"] +
+                   (["filter"] if with_filter else []), check=True, capture_output=True, timeout=600)
     oa, ob, _, _, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb)
     assert st.tie_band == 0 and len(oa) > 0
     assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)) == \
         open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
+    if not with_filter:  # the group at length 50 with altnegative is beyond the oracle's closure
+        return
     fa, fb, _ = gpu_ctx.filter(order, c2, oa, ob)
     assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(fa, fb)) == \
         open(os.path.join(wd, "%d-unique" % order)).read()
+
+
+def test_uncompress_side_v50_full_shape(gpu_ctx):
+    """Single sequences of the v=50 5-compression level expanded 5 -> 1 (length 50:
+    two M-tiles, fixed channels k = 5m dropped, single-split positions folded):
+    gen candidates with the fewest leaves, DFS candidate lines vs the oracle."""
+    split = {1: 10, -1: 10, 3: 5, -3: 5, 5: 1, -5: 1}
+    a, b, _ = gpu_ctx.generate(50, 5, 0)
+    picked = []
+    for cands, seqflag in ((a, 0), (b, 1)):
+        est = [(int(np.prod([split[int(x)] for x in w[1:]])), i) for i, w in enumerate(cands.words)]
+        picked += [(cands.words[i], seqflag) for _, i in sorted(est)[:1]]  # about 2.5e6 leaves, ~30 s in the oracle
+    total = 0
+    for seq, seqflag in picked:
+        ref_text, ref_leaves = _oracle.uncompress_side(seq, 5, 1, 0, seqflag)
+        cands, rot, st = gpu_ctx.uncompress_side(seq, 5, 1, 0, seqflag)
+        assert "".join(l + "\n" for l in cands.lines()) == ref_text, (list(seq), seqflag)
+        assert st.psd_decisions == ref_leaves
+        total += len(cands)
+    assert total > 0  # some expansions pass the filter
