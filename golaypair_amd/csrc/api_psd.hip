@@ -120,6 +120,9 @@ int gp_create(gp_ctx** out, int device) {
         GP_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
         GP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        GP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        for (auto& sl : ctx->slot_ev)
+            for (auto& e : sl) GP_HIP(hipEventCreate(&e));
         GP_HIP(hipEventCreate(&ctx->ev0));
         GP_HIP(hipEventCreate(&ctx->ev1));
         GP_HIP(hipEventCreate(&ctx->ev2));
@@ -138,11 +141,16 @@ void gp_destroy(gp_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev2, ctx->ev3})
         if (e) (void)hipEventDestroy(e);
-    hipStream_t s = ctx->stream;
+    for (auto& sl : ctx->slot_ev)
+        for (hipEvent_t e : sl)
+            if (e) (void)hipEventDestroy(e);
+    hipStream_t s = ctx->stream, s2 = ctx->copy_stream;
     delete ctx;
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
 }
 
 const char* gp_last_error(const gp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }

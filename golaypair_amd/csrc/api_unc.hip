@@ -1,5 +1,6 @@
 // C ABI: uncompression entry points (uncompression.cpp:32-210).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <future>
@@ -105,8 +106,11 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         // (the first batch is a quarter of the budget, so that the device starts
         // while the host is still preparing the second)
         std::vector<std::pair<int64_t, int64_t>> batches;
+        uint64_t leaf_budget = kLeafBudget;
+        if (const char* e = std::getenv("GOLAY_UNC_BATCH_LEAVES"))  // smaller batches (tests)
+            leaf_budget = std::max<uint64_t>(4, std::strtoull(e, nullptr, 10));
         for (int64_t q0 = 0; q0 < P;) {
-            const uint64_t budget = batches.empty() ? kLeafBudget / 4 : kLeafBudget;
+            const uint64_t budget = batches.empty() ? leaf_budget / 4 : leaf_budget;
             uint64_t leaves = 0;
             int64_t q1 = q0;
             while (q1 < P) {
@@ -177,20 +181,16 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             ord_pairs += ord.size();
         };
         std::future<void> ordered;
-        for (size_t bi = 0; bi < batches.size(); ++bi) {
+        // batch i + 1 is queued on the device (in the other buffer slot) before
+        // batch i is collected, so the device never waits for the host between
+        // batches
+        auto collect = [&](UncInflight& F, size_t bi) {
             const auto [q0, q1] = batches[bi];
-            std::shared_ptr<UncPrep> prep = next.get();
-            next = std::move(after);
-            if (bi + 2 < batches.size()) after = std::async(std::launch::async, prepare, bi + 2);
+            auto E = std::make_shared<UncEmitted>();
             UncBatch B;
             {
-                Phase ph("uncompress: filter batch");
-                B = unc_launch(ctx, *prep, &local);
-            }
-            auto E = std::make_shared<UncEmitted>();
-            {
-                Phase ph("uncompress: join batch");
-                *E = unc_join(ctx, B, paf * new_compress, &local);
+                Phase ph("uncompress: collect batch");
+                B = unc_collect(ctx, F, paf * new_compress, *E, &local);
             }
             if (q1 == P && (tmp_a || tmp_b)) {  // the last pair's candidate files
                 last_a = unc_side_cands(ctx, B, (int)(2 * (P - 1 - q0)));
@@ -201,7 +201,21 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 Phase ph("uncompress: order batch");
                 order_batch(*E);
             });
+        };
+        std::shared_ptr<UncInflight> inflight;
+        for (size_t bi = 0; bi < batches.size(); ++bi) {
+            std::shared_ptr<UncPrep> prep = next.get();
+            next = std::move(after);
+            if (bi + 2 < batches.size()) after = std::async(std::launch::async, prepare, bi + 2);
+            std::shared_ptr<UncInflight> F;
+            {
+                Phase ph("uncompress: enqueue batch");
+                F = unc_enqueue(ctx, std::move(prep), (int)(bi & 1), paf * new_compress);
+            }
+            if (inflight) collect(*inflight, bi - 1);
+            inflight = std::move(F);
         }
+        if (inflight) collect(*inflight, batches.size() - 1);
         if (ordered.valid()) ordered.get();
         local.tie_band += ord_ties;
         local.pairs += ord_pairs;

@@ -94,9 +94,13 @@ struct Phase {
     }
     void report() const {
         static const bool on = std::getenv("GOLAY_PROFILE") != nullptr;
-        if (on)
-            std::fprintf(stderr, "[golay] %-28s %9.3f ms\n", name,
-                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        static const auto epoch = std::chrono::steady_clock::now();
+        if (on) {
+            const auto t1 = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[golay] %-28s %9.3f ms  (ends at %.3f ms)\n", name,
+                         std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                         std::chrono::duration<double, std::milli>(t1 - epoch).count());
+        }
     }
 };
 
@@ -165,6 +169,9 @@ struct PinnedPool {
 struct gp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // read-backs that must not wait for later work on `stream`
+    hipEvent_t slot_ev[2][4] = {};      // per batch slot: filter start, filter end, band end, batch end
+    int64_t unc_ecap = 1 << 16;         // emitted-pair rows per in-flight batch (grows on overflow)
     int psd_mode = GP_PSD_SPLIT_BF16;
     std::string err;
     gp::PsdTables tables;

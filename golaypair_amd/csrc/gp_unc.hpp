@@ -92,6 +92,17 @@ void unc_stage(UncPrep& prep, PinnedPool& pool);
 UncBatch unc_filter(gp_ctx* ctx, std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
                     gp_stats* stats);
 UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* stats);
+
+// Asynchronous form of unc_launch + unc_join: unc_enqueue queues the upload,
+// filter, band recheck and join of a batch on the context's stream, in the
+// device buffers of `slot` (0 or 1), with every list length read on the
+// device; unc_collect waits for that batch alone and reads back its emitted
+// pairs.  Batch i + 1 may be enqueued (in the other slot) before batch i is
+// collected; a slot is reused only after its batch is collected.  A batch
+// whose lists overflow is run again by unc_collect on the synchronous path.
+struct UncInflight;
+std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> prep, int slot, int paf_target);
+UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E, gp_stats* stats);
 // Survivors of one input side in DFS order with FP64 keys.
 CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side);
 

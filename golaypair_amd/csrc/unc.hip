@@ -143,8 +143,9 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
 __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
                            const uint64_t* __restrict__ surv, int64_t m, int paf_target, int q_target,
-                           uint64_t* __restrict__ hashes) {
+                           uint64_t* __restrict__ hashes, const unsigned long long* dcount, int64_t dcap) {
     int8_t w[kMaxN];
+    m = list_count(dcount, m, dcap);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t rec = surv[i];
         const UncSide& S = sides[rec >> 40];
@@ -180,7 +181,8 @@ constexpr uint64_t kEmpty = ~0ull;
 
 __global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
                             const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
-                            uint64_t tmask) {
+                            uint64_t tmask, const unsigned long long* dcount, int64_t dcap) {
+    m = list_count(dcount, m, dcap);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         if (sides[surv[i] >> 40].role != 1) continue;
         const uint64_t h = hashes[i];
@@ -199,8 +201,10 @@ __global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __r
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
                            const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
                            uint64_t tmask, int paf_target, int q_target, uint32_t* __restrict__ out_a,
-                           uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl) {
+                           uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl, const unsigned long long* dcount,
+                           int64_t dcap) {
     int8_t wa[kMaxN], wb[kMaxN];
+    m = list_count(dcount, m, dcap);
     unsigned long long checks = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t ra = surv[i];
@@ -242,6 +246,34 @@ __global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __r
         }
     }
     atomicAdd(&ctl->checks, checks);
+}
+
+// Emitted pairs (device count) -> rows [0, ne) the A words/keys, [ne, 2 ne) the B
+// ones, and the pair's group; FP64 keys (A inverted, B plain).
+__global__ void k_emit_pairs(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
+                             const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                             const uint32_t* __restrict__ oa, const uint32_t* __restrict__ ob,
+                             const unsigned long long* __restrict__ dcount, int64_t cap,
+                             const double* __restrict__ trig, int bound, int8_t* __restrict__ words,
+                             uint8_t* __restrict__ keys, int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
+    const int64_t ne = list_count(dcount, 0, cap);
+    double psd[kMaxK];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * ne; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t rec = surv[i < ne ? oa[i] : ob[i - ne]];
+        const UncSide& S = sides[rec >> 40];
+        if (i < ne) groups[i] = S.group;
+        int8_t* w = words + i * S.N;
+        decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, w);
+        fp64_psd(w, S.N, trig, trig + S.N, psd);
+        unsigned int t = 0;
+        for (int k = 0; k < S.K; ++k) {
+            const double f = psd[k] - floor(psd[k]);
+            if (fabs(f - 0.5) < 1e-9) ++t;
+            const int v = (int)rint(psd[k]);
+            keys[i * S.K + k] = (uint8_t)(S.seqflag ? v : bound - v);
+        }
+        if (t) atomicAdd(ties, t);
+    }
 }
 
 // Emitted rows -> words + FP64 keys (A inverted, B plain).
@@ -966,27 +998,28 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
     return prep;
 }
 
-UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
-    UncBatch B;
-    hipStream_t st = ctx->stream;
-    const bool fp32 = U.fp32;
-    const Tables& T = U.T;
-    const Shared& SH = U.SH;
-    const uint64_t total_leaves = U.total_leaves;
+namespace {
+
+void batch_header(const UncPrep& U, UncBatch& B) {
     B.N = U.N;
     B.K = U.N / 2;
     B.trig_off = U.trig_off;
     B.bound = U.bound;
     B.side_of = U.side_of;
-    if (stats) {  // sides rejected by their fixed channels: every leaf decided, none passed
-        stats->words += U.dead_leaves;
-        stats->psd_decisions += U.dead_leaves;
-    }
-    if (T.sides.empty()) return B;
-    Phase ph2("unc_filter: upload");
+    B.nsides_table = (int)U.T.sides.size();
+}
+
+// Survivor and band list capacities for a batch of `leaves` leaves.
+int64_t surv_cap(uint64_t leaves) { return (int64_t)std::min<uint64_t>(leaves * 3 / 5 + (1 << 20), (uint64_t)3 << 30); }
+int64_t band_cap(uint64_t leaves) { return (int64_t)std::min<uint64_t>(leaves / 64 + (1 << 16), (uint64_t)1 << 28); }
+
+// Uploads the staged tables into the device buffer `name` (one copy on the
+// context's stream), points B's table fields there and returns the filter
+// kernels' view of them.
+UncTabs upload_tables(gp_ctx* ctx, UncPrep& U, const std::string& name, UncBatch& B, const int64_t** d_toff) {
     unc_stage(U, ctx->pinned);
-    char* base = static_cast<char*>(ctx->buf("u_stage").ensure(U.stage_bytes));
-    GP_HIP(hipMemcpyAsync(base, U.stage->p, U.stage_bytes, hipMemcpyHostToDevice, st));
+    char* base = static_cast<char*>(ctx->buf(name).ensure(U.stage_bytes));
+    GP_HIP(hipMemcpyAsync(base, U.stage->p, U.stage_bytes, hipMemcpyHostToDevice, ctx->stream));
     auto at = [&](int i) { return static_cast<void*>(base + U.stage_off[i]); };
     B.d_sides = static_cast<const UncSide*>(at(0));
     B.d_radix = static_cast<const int32_t*>(at(1));
@@ -994,20 +1027,72 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     B.d_pm1 = static_cast<const uint64_t*>(at(3));
     B.d_c64 = static_cast<const double*>(at(4));
     B.d_pos64 = static_cast<const int32_t*>(at(5));
-    auto* d_pos32 = static_cast<const int32_t*>(at(6));
-    auto* d_olist = static_cast<const int32_t*>(at(7));
-    auto* d_fix32 = static_cast<const float*>(at(8));
-    auto* d_c32 = static_cast<const float*>(at(9));
-    auto* d_mid = static_cast<const float*>(at(10));
-    auto* d_sufb = static_cast<const uint4*>(at(11));
-    auto* d_frags = static_cast<const uint4*>(at(12));
-    auto* d_ent = static_cast<const int8_t*>(at(13));
-    auto* d_w32 = static_cast<const float*>(at(14));
-    auto* d_toff = static_cast<const int64_t*>(at(15));
-    B.nsides_table = (int)T.sides.size();
+    *d_toff = static_cast<const int64_t*>(at(15));
+    return UncTabs{B.d_sides,
+                   B.d_radix,
+                   static_cast<const int32_t*>(at(6)),
+                   static_cast<const int32_t*>(at(7)),
+                   static_cast<const float*>(at(9)),
+                   static_cast<const float*>(at(10)),
+                   static_cast<const float*>(at(8)),
+                   static_cast<const uint4*>(at(11)),
+                   static_cast<const uint4*>(at(12)),
+                   static_cast<const int8_t*>(at(13)),
+                   static_cast<const float*>(at(14))};
+}
+
+// The filter kernels of one batch on the context's stream.
+void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ctl, uint64_t* d_surv, int64_t scap,
+                    uint64_t* d_band, int64_t bcap) {
+    hipStream_t st = ctx->stream;
+    if (!U.fp32) {
+        for (const UncClass& c : U.classes) {
+            const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 8);
+            const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+            launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
+        }
+    } else {
+        const int64_t waves = std::min<int64_t>(U.T.items, (int64_t)ctx->num_cus * 8);
+        const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+        hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, tb, (int)U.T.sides.size(), U.T.items, ctl, d_surv,
+                           scap, d_band, bcap);
+    }
+    GP_HIP(hipGetLastError());
+}
+
+void add_filter_stats(gp_stats* stats, const UncPrep& U, const UncCtl& h, float ms_filter, float ms_band) {
+    if (!stats) return;
+    stats->words += U.total_leaves;
+    stats->psd_decisions += U.total_leaves;
+    stats->psd_pass += h.surv;
+    stats->band_rechecks += h.band;
+    stats->tie_band += h.ties;
+    stats->psd_kernel_ms += ms_filter;
+    stats->band_kernel_ms += ms_band;
+    stats->psd_launches += U.fp32 ? 1 : U.classes.size();
+    stats->kernel_ms += ms_filter + ms_band;
+}
+
+void add_dead_stats(gp_stats* stats, const UncPrep& U) {
+    if (!stats) return;  // sides rejected by their fixed channels: every leaf decided, none passed
+    stats->words += U.dead_leaves;
+    stats->psd_decisions += U.dead_leaves;
+}
+
+}  // namespace
+
+UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
+    UncBatch B;
+    hipStream_t st = ctx->stream;
+    batch_header(U, B);
+    add_dead_stats(stats, U);
+    if (U.T.sides.empty()) return B;
+    Phase ph2("unc_filter: upload");
+    const int64_t* d_toff = nullptr;
+    const UncTabs tb = upload_tables(ctx, U, "u_stage", B, &d_toff);
     UncCtl* ctl = ctx->dev<UncCtl>("u_ctl", 1);
-    int64_t scap = (int64_t)std::min<uint64_t>(total_leaves * 3 / 5 + (1 << 20), (uint64_t)3 << 30);
-    int64_t bcap = (int64_t)std::min<uint64_t>(total_leaves / 64 + (1 << 16), (uint64_t)1 << 28);
+    int64_t scap = surv_cap(U.total_leaves);
+    int64_t bcap = band_cap(U.total_leaves);
     UncCtl h{};
     float ms_filter = 0, ms_band = 0;
     ph2.next("unc_filter: kernels");
@@ -1016,20 +1101,7 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         uint64_t* d_band = ctx->dev<uint64_t>("u_band", (size_t)bcap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(UncCtl), st));
         GP_HIP(hipEventRecord(ctx->ev0, st));
-        const UncTabs tb{B.d_sides, B.d_radix, d_pos32, d_olist, d_c32, d_mid, d_fix32, d_sufb, d_frags, d_ent, d_w32};
-        if (!fp32) {
-            for (const UncClass& c : U.classes) {
-                const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 8);
-                const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-                launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
-            }
-        } else {
-            const int64_t waves = std::min<int64_t>(T.items, (int64_t)ctx->num_cus * 8);
-            const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-            hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, tb, (int)T.sides.size(), T.items, ctl, d_surv,
-                               scap, d_band, bcap);
-        }
-        GP_HIP(hipGetLastError());
+        launch_filters(ctx, U, tb, ctl, d_surv, scap, d_band, bcap);
         GP_HIP(hipEventRecord(ctx->ev1, st));
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
@@ -1037,7 +1109,8 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         if ((int64_t)h.band > bcap) { bcap = (int64_t)h.band + 1024; continue; }
         if (h.band) {
             hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup,
-                               ctx->tables.d_trig64.as<double>(), d_toff, d_band, ctl, d_surv, scap, (int64_t)h.band);
+                               ctx->tables.d_trig64.as<double>(), d_toff, d_band, ctl, d_surv, scap, (int64_t)h.band,
+                               nullptr, (int64_t)0);
             GP_HIP(hipGetLastError());
             GP_HIP(hipEventRecord(ctx->ev2, st));
             GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
@@ -1049,16 +1122,150 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     }
     B.nsurv = (int64_t)h.surv;
     B.d_surv = ctx->buf("u_surv").as<uint64_t>();
+    add_filter_stats(stats, U, h, ms_filter, ms_band);
+    return B;
+}
+
+// A batch whose filter, band recheck and join are queued on the context's
+// stream with device-side counts, in the buffers of one of two slots.
+struct UncInflight {
+    std::shared_ptr<UncPrep> prep;  // keeps the staged tables for a synchronous rerun
+    int slot = 0;
+    UncBatch B;
+    int64_t scap = 0, bcap = 0, rcap = 0, ecap = 0;
+    UncCtl* ctl = nullptr;
+    JoinCtl2* jctl = nullptr;
+    unsigned int* ties = nullptr;
+    int8_t* w = nullptr;
+    uint8_t* k = nullptr;
+    int32_t* grp = nullptr;
+};
+
+std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> prep, int slot, int paf_target) {
+    auto F = std::make_shared<UncInflight>();
+    F->prep = std::move(prep);
+    F->slot = slot;
+    UncPrep& U = *F->prep;
+    UncBatch& B = F->B;
+    batch_header(U, B);
+    if (U.T.sides.empty()) return F;
+    hipStream_t st = ctx->stream;
+    hipEvent_t* ev = ctx->slot_ev[slot];
+    const std::string sfx = "#" + std::to_string(slot);
+    const int64_t* d_toff = nullptr;
+    const UncTabs tb = upload_tables(ctx, U, "u_stage" + sfx, B, &d_toff);
+    F->scap = surv_cap(U.total_leaves);
+    F->bcap = band_cap(U.total_leaves);
+    F->ctl = ctx->dev<UncCtl>("u_ctl" + sfx, 1);
+    uint64_t* d_surv = ctx->dev<uint64_t>("u_surv" + sfx, (size_t)F->scap);
+    uint64_t* d_band = ctx->dev<uint64_t>("u_band" + sfx, (size_t)F->bcap);
+    B.d_surv = d_surv;
+    GP_HIP(hipMemsetAsync(F->ctl, 0, sizeof(UncCtl), st));
+    GP_HIP(hipEventRecord(ev[0], st));
+    launch_filters(ctx, U, tb, F->ctl, d_surv, F->scap, d_band, F->bcap);
+    GP_HIP(hipEventRecord(ev[1], st));
+    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup,
+                       ctx->tables.d_trig64.as<double>(), d_toff, d_band, F->ctl, d_surv, F->scap, (int64_t)0,
+                       &F->ctl->band, F->bcap);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipEventRecord(ev[2], st));
+    // join over at most rcap survivors (the batch is joined again synchronously
+    // when more survive); the table stays at most half full
+    F->rcap = std::min<int64_t>(F->scap, (int64_t)(U.total_leaves / 32) + (1 << 20));
+    uint64_t T = 1;
+    while (T < (uint64_t)F->rcap * 2) T <<= 1;
+    const int q_target = B.bound + paf_target;
+    uint64_t* hashes = ctx->dev<uint64_t>("u_hash" + sfx, (size_t)F->rcap);
+    unsigned long long* table = ctx->dev<unsigned long long>("u_ht" + sfx, (size_t)T);
+    F->jctl = ctx->dev<JoinCtl2>("u_jctl" + sfx, 1);
+    F->ecap = ctx->unc_ecap;
+    if (const char* e = std::getenv("GOLAY_UNC_EMIT_CAP"))  // fixed emitted-pair capacity (tests)
+        F->ecap = std::max<int64_t>(1, std::strtoll(e, nullptr, 10));
+    uint32_t* oa = ctx->dev<uint32_t>("u_oa" + sfx, (size_t)F->ecap);
+    uint32_t* ob = ctx->dev<uint32_t>("u_ob" + sfx, (size_t)F->ecap);
+    F->w = ctx->dev<int8_t>("u_ew" + sfx, (size_t)2 * F->ecap * B.N);
+    F->k = ctx->dev<uint8_t>("u_ek" + sfx, (size_t)2 * F->ecap * B.K);
+    F->grp = ctx->dev<int32_t>("u_egrp" + sfx, (size_t)F->ecap);
+    F->ties = ctx->dev<unsigned int>("u_eties" + sfx, 1);
+    GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
+    GP_HIP(hipMemsetAsync(F->jctl, 0, sizeof(JoinCtl2), st));
+    GP_HIP(hipMemsetAsync(F->ties, 0, 4, st));
+    const unsigned long long* nsurv = &F->ctl->surv;
+    const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, d_surv,
+                       (int64_t)0, paf_target, q_target, hashes, nsurv, F->rcap);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
+                       nsurv, F->rcap);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv, hashes,
+                       (int64_t)0, table, T - 1, paf_target, q_target, oa, ob, F->ecap, F->jctl, nsurv, F->rcap);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_emit_pairs, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
+                       oa, ob, &F->jctl->out, F->ecap, ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, F->w,
+                       F->k, F->grp, F->ties);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipEventRecord(ev[3], st));
+    return F;
+}
+
+UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E, gp_stats* stats) {
+    UncPrep& U = *F.prep;
+    UncBatch B = F.B;
+    E = UncEmitted{};
+    E.N = B.N;
+    E.K = B.K;
+    if (U.T.sides.empty()) {
+        add_dead_stats(stats, U);
+        return B;
+    }
+    hipEvent_t* ev = ctx->slot_ev[F.slot];
+    GP_HIP(hipEventSynchronize(ev[3]));
+    UncCtl h{};
+    JoinCtl2 hj{};
+    unsigned int ht = 0;
+    hipStream_t cs = ctx->copy_stream;  // does not wait for the next batch, queued on ctx->stream
+    GP_HIP(hipMemcpyAsync(&h, F.ctl, sizeof h, hipMemcpyDeviceToHost, cs));
+    GP_HIP(hipMemcpyAsync(&hj, F.jctl, sizeof hj, hipMemcpyDeviceToHost, cs));
+    GP_HIP(hipMemcpyAsync(&ht, F.ties, 4, hipMemcpyDeviceToHost, cs));
+    GP_HIP(hipStreamSynchronize(cs));
+    if ((int64_t)h.band > F.bcap || (int64_t)h.surv > F.scap) {
+        // a list overflowed: run the whole batch again on the synchronous path
+        B = unc_launch(ctx, U, stats);
+        E = unc_join(ctx, B, paf_target, stats);
+        return B;
+    }
+    float ms_filter = 0, ms_band = 0, ms_join = 0;
+    GP_HIP(hipEventElapsedTime(&ms_filter, ev[0], ev[1]));
+    GP_HIP(hipEventElapsedTime(&ms_band, ev[1], ev[2]));
+    GP_HIP(hipEventElapsedTime(&ms_join, ev[2], ev[3]));
+    add_dead_stats(stats, U);
+    add_filter_stats(stats, U, h, ms_filter, ms_band);
+    if (std::getenv("GOLAY_PROFILE"))
+        std::fprintf(stderr, "[golay] batch: %llu leaves, %llu survivors; device filter %.3f, band %.3f, join %.3f ms\n",
+                     (unsigned long long)U.total_leaves, (unsigned long long)h.surv, ms_filter, ms_band, ms_join);
+    B.nsurv = (int64_t)h.surv;
+    if ((int64_t)h.surv > F.rcap || (int64_t)hj.out > F.ecap) {
+        // the join's lists were too small: join again with exact sizes
+        if ((int64_t)hj.out > F.ecap) ctx->unc_ecap = std::max<int64_t>(ctx->unc_ecap, (int64_t)hj.out * 2);
+        E = unc_join(ctx, B, paf_target, stats);
+        return B;
+    }
+    const int64_t ne = (int64_t)hj.out;
+    E.count = ne;
+    if (ne > 0) {
+        E.words.resize((size_t)2 * ne * B.N);
+        E.keys.resize((size_t)2 * ne * B.K);
+        E.group.resize((size_t)ne);
+        GP_HIP(hipMemcpyAsync(E.words.data(), F.w, E.words.size(), hipMemcpyDeviceToHost, cs));
+        GP_HIP(hipMemcpyAsync(E.keys.data(), F.k, E.keys.size(), hipMemcpyDeviceToHost, cs));
+        GP_HIP(hipMemcpyAsync(E.group.data(), F.grp, E.group.size() * 4, hipMemcpyDeviceToHost, cs));
+        GP_HIP(hipStreamSynchronize(cs));
+    }
     if (stats) {
-        stats->words += total_leaves;
-        stats->psd_decisions += total_leaves;
-        stats->psd_pass += (uint64_t)B.nsurv;
-        stats->band_rechecks += h.band;
-        stats->tie_band += h.ties;
-        stats->psd_kernel_ms += ms_filter;
-        stats->band_kernel_ms += ms_band;
-        stats->psd_launches += fp32 ? 1 : U.classes.size();
-        stats->kernel_ms += ms_filter + ms_band;
+        stats->tie_band += ht;
+        stats->pair_checks += hj.checks;
+        stats->kernel_ms += ms_join;
     }
     return B;
 }
@@ -1089,9 +1296,10 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
     const unsigned grid = (unsigned)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
-                       paf_target, q_target, hashes);
+                       paf_target, q_target, hashes, nullptr, (int64_t)0);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1);
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, nullptr,
+                       (int64_t)0);
     GP_HIP(hipGetLastError());
     int64_t cap = 1 << 16;
     JoinCtl2 h{};
@@ -1100,7 +1308,7 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
         uint32_t* ob = ctx->dev<uint32_t>("u_ob", (size_t)cap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
         hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_surv, hashes,
-                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl);
+                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl, nullptr, (int64_t)0);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));

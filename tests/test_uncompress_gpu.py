@@ -77,6 +77,27 @@ def test_uncompress_pipeline_matches_oracle(gpu_ctx, order, compress, paf, mode)
     assert st.tie_band == 0
 
 
+@pytest.mark.parametrize("batch_leaves,emit_cap", [(64, None), (1 << 12, 1), (1 << 16, 3)])
+def test_uncompress_batches_and_overflow(gpu_ctx, monkeypatch, batch_leaves, emit_cap):
+    """Many small device batches (one input pair each at 64 leaves, so batch
+    i + 1 is queued while batch i is collected) and emitted-pair lists too small
+    for a batch (which is joined again with exact sizes): same pairs, same order."""
+    order, compress, paf = 26, [2, 1], 0
+    wd = _oracle_dir(order, compress, paf, ["gen", "sort", "match", "uncompress"])
+    c, c2 = compress
+    pa, pb = _read_pairs(os.path.join(wd, "%d-pairs-%d" % (order, c)), order // c)
+    assert len(pa) > 4
+    monkeypatch.setenv("GOLAY_UNC_BATCH_LEAVES", str(batch_leaves))
+    if emit_cap is not None:
+        monkeypatch.setenv("GOLAY_UNC_EMIT_CAP", str(emit_cap))
+    oa, ob, ta, tb, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb, want_tmp=True)
+    got = "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob))
+    assert got == open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
+    assert "".join(l + "\n" for l in ta.lines()) == open(os.path.join(wd, "uncompress_tmp_a")).read()
+    assert "".join(l + "\n" for l in tb.lines()) == open(os.path.join(wd, "uncompress_tmp_b")).read()
+    assert st.tie_band == 0
+
+
 def test_uncompress_empty_input(gpu_ctx):
     z = np.zeros((0, 8), np.int8)
     oa, ob, ta, tb, st = gpu_ctx.uncompress(16, 2, 1, 0, z, z, want_tmp=True)
