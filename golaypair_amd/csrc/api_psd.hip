@@ -99,7 +99,7 @@ __global__ void k_mb_digest(uint64_t seed, uint64_t count, const MbKind* __restr
             atomicAdd(&accepted[k], 1ull);
         }
     }
-    atomicAdd(digest, dsum);
+    wave_atomic_add(digest, dsum);
 }
 
 }  // namespace

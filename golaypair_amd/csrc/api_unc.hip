@@ -25,17 +25,6 @@ namespace {
 
 constexpr uint64_t kLeafBudget = 1ull << 30;  // leaves per device batch
 
-// Builds both sides of pairs [q0, q1) on host threads.
-void build_sides(const PairSet& inp, int L, int c, int c2, int64_t q0, int64_t q1, std::vector<SideHost>& out) {
-    const int64_t n = q1 - q0;
-    out.resize(2 * n);
-    const auto tab = make_splits(c, c2);
-    pool().parallel_for((size_t)n, [&](size_t i) {
-        out[2 * i] = build_side(tab, &inp.a[(q0 + i) * L], L, 0);
-        out[2 * i + 1] = build_side(tab, &inp.b[(q0 + i) * L], L, 1);
-    });
-}
-
 }  // namespace
 
 extern "C" {
@@ -95,51 +84,76 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         CandSet last_a, last_b;
         last_a.n = last_b.n = N;
         last_a.nkey = last_b.nkey = N / 2;
-        // build all sides up front (host threads), then batch by leaf budget
-        std::vector<SideHost> all;
-        {
-            Phase ph("uncompress: build sides");
-            build_sides(inp, L, compress, new_compress, 0, P, all);
-        }
-        // batches by leaf budget; the host tables of batch i+1 are prepared on
-        // another thread while the device runs batch i
-        // (the first batch is a quarter of the budget, so that the device starts
-        // while the host is still preparing the second)
-        std::vector<std::pair<int64_t, int64_t>> batches;
+        // Sides are built on host threads, the first batch's first, so that its
+        // tables are prepared while the other sides are built.  Batches by leaf
+        // budget; the host tables of batch i + 2 are prepared on another thread
+        // while the device runs batch i (the first batch is a quarter of the
+        // budget, so that the device starts while the host prepares the second).
+        std::vector<SideHost> all((size_t)(2 * P));
+        std::vector<int> group(all.size());
+        for (size_t i = 0; i < all.size(); ++i) group[i] = (int)(i / 2);
         uint64_t leaf_budget = kLeafBudget;
         if (const char* e = std::getenv("GOLAY_UNC_BATCH_LEAVES"))  // smaller batches (tests)
             leaf_budget = std::max<uint64_t>(4, std::strtoull(e, nullptr, 10));
-        for (int64_t q0 = 0; q0 < P;) {
-            const uint64_t budget = batches.empty() ? leaf_budget / 4 : leaf_budget;
-            uint64_t leaves = 0;
-            int64_t q1 = q0;
-            while (q1 < P) {
-                const uint64_t add = all[2 * q1].leaves + all[2 * q1 + 1].leaves;
-                if (q1 > q0 && leaves + add > budget) break;
-                leaves += add;
-                ++q1;
-            }
-            batches.emplace_back(q0, q1);
-            q0 = q1;
-        }
-        std::vector<int> group(all.size());
-        for (size_t i = 0; i < all.size(); ++i) group[i] = (int)(i / 2);
         UncTrig trig;
         if (P > 0) trig = unc_trig(ctx, N);
         const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
-        auto prepare = [&](size_t i) {
-            const auto [q0, q1] = batches[i];
+        auto prepare = [&, L](int64_t q0, int64_t q1) {
             const std::span<const SideHost> sides(all.data() + 2 * q0, (size_t)(2 * (q1 - q0)));
             const std::span<const int> grp(group.data(), sides.size());
-            auto prep = unc_prepare(sides, grp, order, paf, fp32, trig);
-            GP_HIP(hipSetDevice(ctx->device));
-            unc_stage(*prep, ctx->pinned);
-            return prep;
+            GP_HIP(hipSetDevice(ctx->device));  // the pinned staging buffer
+            return unc_prepare(sides, grp, order, paf, fp32, trig, &ctx->pinned);
         };
-        // the first two batches are prepared together; batch i + 2 while batch i runs
+        const auto tab = make_splits(compress, new_compress);
+        auto build_range = [&](int64_t q0, int64_t q1) {
+            pool().parallel_for((size_t)(q1 - q0), [&](size_t i) {
+                const int64_t q = q0 + (int64_t)i;
+                all[2 * q] = build_side(tab, &inp.a[q * L], L, 0);
+                all[2 * q + 1] = build_side(tab, &inp.b[q * L], L, 1);
+            });
+        };
+        auto pair_leaves = [&](int64_t q) { return all[2 * q].leaves + all[2 * q + 1].leaves; };
+        std::vector<std::pair<int64_t, int64_t>> batches;
         std::future<std::shared_ptr<UncPrep>> next, after;
-        if (!batches.empty()) next = std::async(std::launch::async, prepare, 0);
-        if (batches.size() > 1) after = std::async(std::launch::async, prepare, 1);
+        {
+            Phase ph("uncompress: build sides");
+            int64_t built = 0, q1 = 0;
+            uint64_t leaves = 0;
+            bool closed = P == 0;
+            while (!closed && built < P) {  // the first batch, in chunks of pairs
+                const int64_t c1 = std::min<int64_t>(P, built + 512);
+                build_range(built, c1);
+                built = c1;
+                for (; q1 < built; ++q1) {
+                    const uint64_t add = pair_leaves(q1);
+                    if (q1 > 0 && leaves + add > leaf_budget / 4) {
+                        closed = true;
+                        break;
+                    }
+                    leaves += add;
+                }
+                closed = closed || q1 == P;
+            }
+            if (P > 0) {
+                batches.emplace_back(0, q1);
+                next = std::async(std::launch::async, prepare, (int64_t)0, q1);
+            }
+            build_range(built, P);
+            for (int64_t q0 = q1; q0 < P;) {
+                uint64_t lv = 0;
+                int64_t e = q0;
+                while (e < P) {
+                    const uint64_t add = pair_leaves(e);
+                    if (e > q0 && lv + add > leaf_budget) break;
+                    lv += add;
+                    ++e;
+                }
+                batches.emplace_back(q0, e);
+                q0 = e;
+            }
+            if (batches.size() > 1)
+                after = std::async(std::launch::async, prepare, batches[1].first, batches[1].second);
+        }
         // Emitted pairs of one batch in the reference's order (input pair, then A
         // line text, then B line text), appended to `result`.  Runs on a host
         // thread while the device works on the next batch; tasks are chained so
@@ -206,7 +220,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         for (size_t bi = 0; bi < batches.size(); ++bi) {
             std::shared_ptr<UncPrep> prep = next.get();
             next = std::move(after);
-            if (bi + 2 < batches.size()) after = std::async(std::launch::async, prepare, bi + 2);
+            if (bi + 2 < batches.size())
+                after = std::async(std::launch::async, prepare, batches[bi + 2].first, batches[bi + 2].second);
             std::shared_ptr<UncInflight> F;
             {
                 Phase ph("uncompress: enqueue batch");

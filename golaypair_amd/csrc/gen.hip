@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
             }
         }
     }
-    atomicAdd(&ctl->words, nwords);
+    wave_atomic_add(&ctl->words, nwords);
 }
 
 __global__ void k_gen_unpack(const u128* __restrict__ surv_w, int64_t m, int n, int a0, int stride,
@@ -239,11 +239,20 @@ __global__ void k_gen_emit(const int8_t* __restrict__ words, int stride, int64_t
                            unsigned int* __restrict__ ties) {
     const int K = n >> 1;
     double psd[kMaxK];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        if (decision[i] != kAccept) continue;
+    const int lane = threadIdx.x & 63;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool acc = i < m && decision[i] == kAccept;
+        // one atomic per wave for the wave's accepted rows (slot < m)
+        const unsigned long long bal = __ballot(acc);
+        if (!bal) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(count, (unsigned long long)__popcll(bal));
+        base = __shfl(base, 0);
+        if (!acc) continue;
+        const unsigned long long slot = base + __popcll(bal & ((1ull << lane) - 1ull));
         const int8_t* w = words + i * stride;
         fp64_psd(w, n, trig, trig + n, psd);
-        const unsigned long long slot = atomicAdd(count, 1ull);  // slot < m: one per accepted row
         unsigned int t = 0;
         for (int k = 0; k < K; ++k) {
             const double f = psd[k] - floor(psd[k]);

@@ -79,13 +79,14 @@ UncTrig unc_trig(gp_ctx* ctx, int N);
 // Host tables of a batch of sides.  unc_prepare touches no device state and may
 // run on another host thread while the device works on the previous batch;
 // unc_launch uploads the tables and runs the filter (unc_filter = both).
+// Every table is written in place into one staging buffer: a pinned buffer
+// from `pool` when given, else host memory of the batch (then unc_stage copies
+// it into a pinned buffer before the upload).
 struct UncPrep;
-std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
-                                     bool fp32, const UncTrig& trig);
-UncBatch unc_launch(gp_ctx* ctx, UncPrep& prep, gp_stats* stats);
-// Copies a prepared batch's tables into one pinned staging buffer from `pool`
-// (host memory only; unc_launch stages a batch that was not staged).
 struct PinnedPool;
+std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
+                                     bool fp32, const UncTrig& trig, PinnedPool* pool = nullptr);
+UncBatch unc_launch(gp_ctx* ctx, UncPrep& prep, gp_stats* stats);
 void unc_stage(UncPrep& prep, PinnedPool& pool);
 
 // group[i]: pair index of input side i (a-side seqflag 0, b-side seqflag 1).

@@ -112,6 +112,29 @@ __device__ inline double fp64_max_psd(const int8_t* __restrict__ w, int n, const
     return mx;
 }
 
+// Adds the sum of `v` over the lanes of the calling wave to *p with one
+// atomic (none when the sum is 0).  Every lane of the wave must call it.
+__device__ inline void wave_atomic_add(unsigned long long* p, unsigned long long v) {
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(p, v);
+}
+
+// FP64 PSD_k of one frequency k, summed in the order of fp64_psd and
+// fp64_max_psd (so all three agree bit for bit).
+__device__ inline double fp64_psd_k(const int8_t* __restrict__ w, int n, const double* __restrict__ c,
+                                    const double* __restrict__ s, int k) {
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int p = 0; p < n; ++p) {
+        double x = (double)w[p];
+        re = __builtin_fma(x, c[m], re);
+        im = __builtin_fma(x, s[m], im);
+        m += k;
+        if (m >= n) m -= n;
+    }
+    return re * re + im * im;
+}
+
 // FP64 PSD_k for k = 1..K into out[0..K-1].
 __device__ inline void fp64_psd(const int8_t* __restrict__ w, int n, const double* __restrict__ c,
                                 const double* __restrict__ s, double* out) {

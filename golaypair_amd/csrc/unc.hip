@@ -14,6 +14,7 @@
 // FP64 keys are computed for the emitted pairs only (their output order is the
 // text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
 // of each emitted pair is checked (counted in tie_band when it fails).
+#include <array>
 #include <functional>
 #include <numeric>
 #include <tuple>
@@ -140,15 +141,34 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
     return mask;
 }
 
+struct JoinCtl2 {
+    unsigned long long out;     // emitted pairs
+    unsigned long long checks;  // full PAF checks
+    unsigned long long nb;      // B rows (counted by k_unc_rows when `jctl` is given)
+};
+
+// Hash table size in use: the host's `tmask` + 1, or, with `jctl`, the
+// smallest power of two >= 2 * (B rows) (at least 64, at most tmask + 1).
+__device__ __forceinline__ uint64_t table_mask(const JoinCtl2* jctl, uint64_t tmask) {
+    if (!jctl) return tmask;
+    const uint64_t want = 2 * jctl->nb;
+    uint64_t T = 64;
+    while (T < want && T <= tmask) T <<= 1;
+    return T - 1;
+}
+
 __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
                            const uint64_t* __restrict__ surv, int64_t m, int paf_target, int q_target,
-                           uint64_t* __restrict__ hashes, const unsigned long long* dcount, int64_t dcap) {
+                           uint64_t* __restrict__ hashes, const unsigned long long* dcount, int64_t dcap,
+                           JoinCtl2* jctl) {
     int8_t w[kMaxN];
     m = list_count(dcount, m, dcap);
+    unsigned long long nb = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t rec = surv[i];
         const UncSide& S = sides[rec >> 40];
+        nb += S.role == 1;
         const int N = S.N;
         const bool fast = S.pm1_off >= 0;
         const uint64_t mask = decode_word(S, rec & ((1ull << 40) - 1), radix, tup, pm1, fast ? nullptr : w);
@@ -175,14 +195,23 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
         }
         hashes[i] = splitmix64(h);
     }
+    if (jctl) wave_atomic_add(&jctl->nb, nb);
 }
 
 constexpr uint64_t kEmpty = ~0ull;
 
+// Empties the part of the table in use (see table_mask).
+__global__ void k_ht_fill(unsigned long long* __restrict__ table, uint64_t tmask, const JoinCtl2* jctl) {
+    const uint64_t T = table_mask(jctl, tmask) + 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (uint64_t)gridDim.x * blockDim.x)
+        table[i] = ~0ull;
+}
+
 __global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
                             const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
-                            uint64_t tmask, const unsigned long long* dcount, int64_t dcap) {
+                            uint64_t tmask, const unsigned long long* dcount, int64_t dcap, const JoinCtl2* jctl) {
     m = list_count(dcount, m, dcap);
+    tmask = table_mask(jctl, tmask);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         if (sides[surv[i] >> 40].role != 1) continue;
         const uint64_t h = hashes[i];
@@ -192,19 +221,15 @@ __global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* _
     }
 }
 
-struct JoinCtl2 {
-    unsigned long long out;
-    unsigned long long checks;
-};
-
 __global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
                            const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
                            uint64_t tmask, int paf_target, int q_target, uint32_t* __restrict__ out_a,
                            uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl, const unsigned long long* dcount,
-                           int64_t dcap) {
+                           int64_t dcap, bool device_mask) {
     int8_t wa[kMaxN], wb[kMaxN];
     m = list_count(dcount, m, dcap);
+    tmask = table_mask(device_mask ? ctl : nullptr, tmask);
     unsigned long long checks = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t ra = surv[i];
@@ -245,60 +270,72 @@ __global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __r
             slot = (slot + 1) & tmask;
         }
     }
-    atomicAdd(&ctl->checks, checks);
+    wave_atomic_add(&ctl->checks, checks);
 }
 
-// Emitted pairs (device count) -> rows [0, ne) the A words/keys, [ne, 2 ne) the B
-// ones, and the pair's group; FP64 keys (A inverted, B plain).
-__global__ void k_emit_pairs(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
-                             const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
-                             const uint32_t* __restrict__ oa, const uint32_t* __restrict__ ob,
-                             const unsigned long long* __restrict__ dcount, int64_t cap,
-                             const double* __restrict__ trig, int bound, int8_t* __restrict__ words,
-                             uint8_t* __restrict__ keys, int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
+// Emitted rows -> words + FP64 key digits (A inverted, B plain) + group of
+// rows i < ngroups.  Blocks of one wave, one row per half-wave: the row's word
+// is decoded into LDS, lane q of the half computes PSD_{q+1} (the order of
+// fp64_psd) and writes bytes q and q + 32 of the word.
+template <class RecOf>
+__device__ __forceinline__ void emit_rows_halfwave(int64_t m, int64_t ngroups, RecOf rec_of,
+                                                   const UncSide* __restrict__ sides,
+                                                   const int32_t* __restrict__ radix, const int8_t* __restrict__ tup,
+                                                   const double* __restrict__ trig, int bound,
+                                                   int8_t* __restrict__ words, uint8_t* __restrict__ keys,
+                                                   int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
+    __shared__ int8_t wl[2][kMaxN];
+    const int h = threadIdx.x >> 5, q = threadIdx.x & 31;
+    for (int64_t i0 = 2 * (int64_t)blockIdx.x; i0 < m; i0 += 2 * (int64_t)gridDim.x) {
+        const int64_t i = i0 + h;
+        const bool live = i < m;
+        const uint64_t rec = live ? rec_of(i) : 0;
+        const UncSide& S = sides[rec >> 40];
+        if (live && q == 0) decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, wl[h]);
+        __syncthreads();
+        if (live) {
+            const int N = S.N;
+            if (q < N) words[i * N + q] = wl[h][q];
+            if (q + 32 < N) words[i * N + q + 32] = wl[h][q + 32];
+            if (q == 0 && i < ngroups) groups[i] = S.group;
+            if (q < S.K) {
+                const double p = fp64_psd_k(wl[h], N, trig, trig + N, q + 1);
+                const double f = p - floor(p);
+                if (fabs(f - 0.5) < 1e-9) atomicAdd(ties, 1u);
+                const int v = (int)rint(p);
+                keys[i * S.K + q] = (uint8_t)(S.seqflag ? v : bound - v);
+            }
+        }
+        __syncthreads();  // wl is rewritten by the next iteration
+    }
+}
+
+// Emitted pairs (device count) -> rows [0, ne) the A words/keys, [ne, 2 ne) the
+// B ones, and the pair's group.
+__global__ void __launch_bounds__(64) k_emit_pairs(const UncSide* __restrict__ sides,
+                                                   const int32_t* __restrict__ radix,
+                                                   const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                                                   const uint32_t* __restrict__ oa, const uint32_t* __restrict__ ob,
+                                                   const unsigned long long* __restrict__ dcount, int64_t cap,
+                                                   const double* __restrict__ trig, int bound,
+                                                   int8_t* __restrict__ words, uint8_t* __restrict__ keys,
+                                                   int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
     const int64_t ne = list_count(dcount, 0, cap);
-    double psd[kMaxK];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * ne; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t rec = surv[i < ne ? oa[i] : ob[i - ne]];
-        const UncSide& S = sides[rec >> 40];
-        if (i < ne) groups[i] = S.group;
-        int8_t* w = words + i * S.N;
-        decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, w);
-        fp64_psd(w, S.N, trig, trig + S.N, psd);
-        unsigned int t = 0;
-        for (int k = 0; k < S.K; ++k) {
-            const double f = psd[k] - floor(psd[k]);
-            if (fabs(f - 0.5) < 1e-9) ++t;
-            const int v = (int)rint(psd[k]);
-            keys[i * S.K + k] = (uint8_t)(S.seqflag ? v : bound - v);
-        }
-        if (t) atomicAdd(ties, t);
-    }
+    emit_rows_halfwave(
+        2 * ne, ne, [&](int64_t i) { return surv[i < ne ? oa[i] : ob[i - ne]]; }, sides, radix, tup, trig, bound,
+        words, keys, groups, ties);
 }
 
-// Emitted rows -> words + FP64 keys (A inverted, B plain).
-__global__ void k_emit_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
-                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
-                            const uint32_t* __restrict__ rows, int64_t m, const double* __restrict__ trig, int bound,
-                            int8_t* __restrict__ words, uint8_t* __restrict__ keys, int32_t* __restrict__ groups,
-                            unsigned int* __restrict__ ties) {
-    double psd[kMaxK];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t rec = surv[rows[i]];
-        const UncSide& S = sides[rec >> 40];
-        groups[i] = S.group;
-        int8_t* w = words + i * S.N;
-        decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, w);
-        fp64_psd(w, S.N, trig, trig + S.N, psd);
-        unsigned int t = 0;
-        for (int k = 0; k < S.K; ++k) {
-            const double f = psd[k] - floor(psd[k]);
-            if (fabs(f - 0.5) < 1e-9) ++t;
-            const int v = (int)rint(psd[k]);
-            keys[i * S.K + k] = (uint8_t)(S.seqflag ? v : bound - v);
-        }
-        if (t) atomicAdd(ties, t);
-    }
+// Emitted rows (surv[rows[i]]) -> words, FP64 keys and groups.
+__global__ void __launch_bounds__(64) k_emit_rows(const UncSide* __restrict__ sides,
+                                                  const int32_t* __restrict__ radix,
+                                                  const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                                                  const uint32_t* __restrict__ rows, int64_t m,
+                                                  const double* __restrict__ trig, int bound,
+                                                  int8_t* __restrict__ words, uint8_t* __restrict__ keys,
+                                                  int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
+    emit_rows_halfwave(
+        m, m, [&](int64_t i) { return surv[rows[i]]; }, sides, radix, tup, trig, bound, words, keys, groups, ties);
 }
 
 // Records of one table side (unordered) -> compact list.
@@ -328,15 +365,18 @@ double bf16_d(uint16_t h) {
     return f;
 }
 
-// Per-side tables (concatenated over the sides of a call).
+// Per-side tables (concatenated over the sides of a call), views into the
+// batch's staging buffer.
 struct Tables {
-    std::vector<UncSide> sides;
-    std::vector<int32_t> radix;
-    std::vector<int8_t> tup;
-    std::vector<uint64_t> pm1;
-    std::vector<int32_t> pos32, pos64;  // per side and position: block offsets into Shared
-    std::vector<int32_t> olist;         // per side: outer positions (active, in order)
-    std::vector<float> fix32;           // per side: FP32 sum of the single-split positions
+    int64_t nsides = 0;
+    UncSide* sides = nullptr;
+    int32_t* radix = nullptr;
+    int8_t* tup = nullptr;
+    uint64_t* pm1 = nullptr;
+    int32_t* pos32 = nullptr;  // per side and position: block offsets into Shared
+    int32_t* pos64 = nullptr;
+    int32_t* olist = nullptr;  // per side: outer positions (active, in order)
+    float* fix32 = nullptr;    // per side: FP32 sum of the single-split positions
     int64_t items = 0;
 };
 
@@ -351,11 +391,15 @@ struct Shared {
     std::vector<double> c64;
     std::map<std::vector<int>, int64_t> frag_off;  // suffix positions -> offset into frags
     std::map<std::vector<int>, double> rho;        // suffix positions -> max BF16 split residual
-    std::vector<uint4> frags;
-    std::vector<float> mid, w32;      // mid tables; FP32 suffix weights (FP32 path)
-    std::vector<uint4> sufb;          // suffix B operands (split path)
-    std::vector<int8_t> ent;          // suffix entries (FP32 path)
-    std::map<std::vector<int>, int64_t> w32_off;  // suffix positions -> offset into w32
+    std::map<std::vector<int>, int64_t> w32_off;   // suffix positions -> offset into w32
+    std::vector<std::vector<int>> frag_sets, w32_sets;  // in offset order
+    int64_t n_frags = 0, n_w32 = 0, n_mid = 0, n_suf = 0;
+    // views into the staging buffer
+    uint4* frags = nullptr;  // MFMA A fragments per suffix position set (split path)
+    float* w32 = nullptr;    // FP32 suffix weights (FP32 path)
+    float* mid = nullptr;    // mid tables
+    uint4* sufb = nullptr;   // suffix B operands (split path)
+    int8_t* ent = nullptr;   // suffix entries (FP32 path)
     // FP32 filter channels: slot q (row q % 16 of M-tile q / 16) holds k = chan[q]
     std::vector<int> chan;
     int mt() const { return ((int)chan.size() + 15) / 16; }
@@ -460,6 +504,12 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, uint4* 
         std::copy(vals.begin(), vals.end(), ent);
         return;
     }
+    // BF16 of every int8 value (exact: |v| <= 128 has at most 8 significant bits)
+    static const std::array<uint16_t, 256> bf = [] {
+        std::array<uint16_t, 256> t{};
+        for (int v = -128; v < 128; ++v) t[v + 128] = bf16_rne_u((float)v);
+        return t;
+    }();
     const int nbs = std::max(1, (nent + 15) / 16);
     const int ntiles = (int)((ps + 31) / 32);
     for (int tile = 0; tile < ntiles; ++tile)
@@ -470,7 +520,7 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, uint4* 
                 if (s < ps)
                     for (int i8 = 0; i8 < 8; ++i8) {
                         const int e = 16 * blk + 8 * (lane / 32) + i8;
-                        if (e < nent) hv[i8] = bf16_rne_u((float)vals[s * nent + e]);
+                        if (e < nent) hv[i8] = bf[vals[s * nent + e] + 128];
                     }
                 uint4 q;
                 q.x = hv[0] | ((uint32_t)hv[1] << 16);
@@ -489,11 +539,19 @@ int64_t suf_size(const Split& X, bool fp32) {
 }
 
 // FP32 suffix DFT weights of the suffix positions, [nent][mt*32] (FP32 path).
-void add_w32(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
-             const std::vector<double>& sn) {
-    if (SH.w32_off.count(sp)) return;
+// Registers the suffix position set `sp` (offset and size in w32).
+void add_w32(Shared& SH, const std::vector<int>& sp, int r) {
+    auto [it, fresh] = SH.w32_off.emplace(sp, SH.n_w32);
+    if (!fresh) return;
+    SH.w32_sets.push_back(sp);
+    SH.n_w32 += (int64_t)sp.size() * r * SH.mt() * 32;
+}
+
+// Writes the weights of a registered set.
+void build_w32(const Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+               const std::vector<double>& sn) {
     const int mt = SH.mt(), nent = (int)sp.size() * r;
-    SH.w32_off[sp] = (int64_t)SH.w32.size();
+    float* out = SH.w32 + SH.w32_off.at(sp);
     for (int e = 0; e < nent; ++e) {
         const int j = sp[e / r], i = e % r, p = j + L * i;
         for (int t = 0; t < mt; ++t)
@@ -504,7 +562,7 @@ void add_w32(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const 
                     const int mm = (int)(((int64_t)SH.chan[q] * p) % N);
                     v = (float)(row >= 16 ? sn[mm] : cs[mm]);
                 }
-                SH.w32.push_back(v);
+                *out++ = v;
             }
     }
 }
@@ -548,11 +606,21 @@ void add_blocks(Shared& SH, const SplitTable& X, int L, int N, const std::vector
         }
 }
 
-void add_frags(Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
-               const std::vector<double>& sn) {
-    if (SH.frag_off.count(sp)) return;
+// Registers the suffix position set `sp` (offset and size in frags).
+void add_frags(Shared& SH, const std::vector<int>& sp, int r) {
+    auto [it, fresh] = SH.frag_off.emplace(sp, SH.n_frags);
+    if (!fresh) return;
+    SH.frag_sets.push_back(sp);
+    const int nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
+    SH.n_frags += (int64_t)SH.mt() * 2 * nbs * 64;
+}
+
+// Writes the fragments of a registered set; returns the largest BF16 split
+// residual |w - hi - lo| of its weights.
+double build_frags(const Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+                   const std::vector<double>& sn) {
     const int mt = SH.mt(), nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
-    SH.frag_off[sp] = (int64_t)SH.frags.size();
+    uint4* out = SH.frags + SH.frag_off.at(sp);
     double rho = 0;
     for (int t = 0; t < mt; ++t)
         for (int st = 0; st < 2 * nbs; ++st)
@@ -583,9 +651,9 @@ void add_frags(Shared& SH, const std::vector<int>& sp, int L, int r, int N, cons
                 q.y = hv[2] | ((uint32_t)hv[3] << 16);
                 q.z = hv[4] | ((uint32_t)hv[5] << 16);
                 q.w = hv[6] | ((uint32_t)hv[7] << 16);
-                SH.frags.push_back(q);
+                *out++ = q;
             }
-    SH.rho[sp] = rho;
+    return rho;
 }
 
 // Where one side's rows go in the batch tables (filled by add_side on host threads).
@@ -730,9 +798,12 @@ struct UncPrep {
     std::vector<UncClass> classes;
     uint64_t total_leaves = 0;
     uint64_t dead_leaves = 0;  // leaves of sides rejected by their fixed channels
-    // every table in one pinned staging buffer (one host-to-device copy)
+    // every table in one staging buffer (one host-to-device copy): pinned
+    // (`stage`) or, when prepared without a pool, `host` until unc_stage
     PinnedBuf* stage = nullptr;
     PinnedPool* stage_pool = nullptr;
+    std::unique_ptr<char[]> host;
+    char* base = nullptr;
     size_t stage_bytes = 0;
     std::vector<size_t> stage_off;
     ~UncPrep() {
@@ -742,43 +813,21 @@ struct UncPrep {
 
 namespace {
 
-struct HostSpan {
-    const void* p;
-    size_t bytes;
+// Staging buffer regions, in upload order (sizes in bytes).
+enum StageRegion {
+    kStSides, kStRadix, kStTup, kStPm1, kStC64, kStPos64, kStPos32, kStOlist, kStFix32, kStC32,
+    kStMid, kStSufb, kStFrags, kStEnt, kStW32, kStToff, kStCount
 };
-
-template <class V>
-HostSpan span_of(const std::vector<V>& v) {
-    return HostSpan{v.data(), v.size() * sizeof(V)};
-}
-
-// The tables of a batch in staging order (the order unc_launch reads them back).
-std::vector<HostSpan> stage_spans(const UncPrep& U, const int64_t* toff) {
-    const Tables& T = U.T;
-    const Shared& SH = U.SH;
-    return {span_of(T.sides), span_of(T.radix), span_of(T.tup),   span_of(T.pm1),   span_of(SH.c64),
-            span_of(T.pos64), span_of(T.pos32), span_of(T.olist), span_of(T.fix32), span_of(SH.c32),
-            span_of(SH.mid),  span_of(SH.sufb), span_of(SH.frags), span_of(SH.ent),  span_of(SH.w32),
-            HostSpan{toff, sizeof(int64_t)}};
-}
 
 }  // namespace
 
 void unc_stage(UncPrep& U, PinnedPool& pool) {
-    if (U.stage || U.T.sides.empty()) return;
-    const int64_t toff = U.trig_off;
-    const std::vector<HostSpan> sp = stage_spans(U, &toff);
-    U.stage_off.assign(sp.size(), 0);
-    size_t total = 0;
-    for (size_t i = 0; i < sp.size(); ++i) {
-        U.stage_off[i] = total;
-        total += (sp[i].bytes + 255) & ~(size_t)255;
-    }
-    U.stage = pool.acquire(total);
+    if (U.stage || U.T.nsides == 0) return;
+    U.stage = pool.acquire(U.stage_bytes);
     U.stage_pool = &pool;
-    U.stage_bytes = total;
-    for (size_t i = 0; i < sp.size(); ++i)
-        if (sp[i].bytes) std::memcpy(static_cast<char*>(U.stage->p) + U.stage_off[i], sp[i].p, sp[i].bytes);
+    std::memcpy(U.stage->p, U.base, U.stage_bytes);
+    U.base = static_cast<char*>(U.stage->p);
+    U.host.reset();
 }
 
 template <int NBS, int MT, bool HALF>
@@ -815,7 +864,7 @@ UncTrig unc_trig(gp_ctx* ctx, int N) {
 }
 
 std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
-                                     bool fp32, const UncTrig& trig) {
+                                     bool fp32, const UncTrig& trig, PinnedPool* pinned) {
     auto prep = std::make_shared<UncPrep>();
     UncPrep& U = *prep;
     if (sides.empty()) return prep;
@@ -849,22 +898,34 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         const int r = sides[0].r, K = N / 2;
         const double thr = 2.0 * order - paf + 0.001;
         bool all_channels = r <= 1 || K < r;
+        auto parallel = [](size_t count, const std::function<void(size_t)>& f) { pool().parallel_for(count, f); };
+        // per side: bit 0 = dead (a fixed channel above the threshold), bit 1 = a
+        // fixed channel within 1e-6 of it
+        std::vector<uint8_t> fixed_state(sides.size(), 0);
+        if (r > 1)
+            parallel(sides.size(), [&](size_t i) {
+                if (sides[i].leaves == 0) return;
+                uint8_t st = 0;
+                for (int k = r; k <= K; k += r) {
+                    double re = 0, im = 0;
+                    int mm = 0;  // k j mod N
+                    for (int j = 0; j < sides[i].L; ++j) {
+                        re += sides[i].orig[j] * cs[mm];
+                        im += sides[i].orig[j] * sn[mm];
+                        mm += k;
+                        if (mm >= N) mm -= N;
+                    }
+                    const double psd = re * re + im * im;
+                    if (psd > thr + 1e-6) st |= 1;
+                    else if (psd >= thr - 1e-6) st |= 2;
+                }
+                fixed_state[i] = st;
+            });
         std::vector<size_t> live;
         for (size_t i = 0; i < sides.size(); ++i) {
             if (sides[i].leaves == 0) continue;
-            bool dead = false;
-            for (int k = r; k <= K && r > 1; k += r) {
-                double re = 0, im = 0;
-                for (int j = 0; j < sides[i].L; ++j) {
-                    const int mm = (int)(((int64_t)k * j) % N);
-                    re += sides[i].orig[j] * cs[mm];
-                    im += sides[i].orig[j] * sn[mm];
-                }
-                const double psd = re * re + im * im;
-                if (psd > thr + 1e-6) dead = true;
-                else if (psd >= thr - 1e-6) all_channels = true;
-            }
-            if (dead) {
+            if (fixed_state[i] & 2) all_channels = true;
+            if (fixed_state[i] & 1) {
                 U.dead_leaves += sides[i].leaves;
                 continue;
             }
@@ -876,9 +937,8 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 if (sides[i].leaves != 0) live.push_back(i);
         for (int k = 1; k <= K; ++k)
             if (all_channels || k % r != 0) SH.chan.push_back(k);
-        auto parallel = [](size_t count, const std::function<void(size_t)>& f) { pool().parallel_for(count, f); };
         // shared per-(position, letter) blocks; per side its split and table keys
-        Phase ph1("tables: blocks + keys");
+        Phase ph1("tables: splits + keys");
         if (!live.empty()) add_blocks(SH, *sides[live[0]].tab, sides[live[0]].L, N, cs, sn);
         std::vector<Split> spl(live.size());
         std::vector<std::string> kmid(live.size()), ksuf(live.size());
@@ -890,10 +950,11 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             ksuf[q] = pos_key(S, spl[q].suf);
             pm1[q] = side_pm1(S);
         });
+        ph1.next("tables: layout");
+        // suffix weight sets (built below, in parallel)
         for (size_t q = 0; q < live.size(); ++q) {
-            const SideHost& S = sides[live[q]];
-            if (!fp32) add_frags(SH, spl[q].suf, S.L, S.r, N, cs, sn);
-            else add_w32(SH, spl[q].suf, S.L, S.r, N, cs, sn);
+            if (!fp32) add_frags(SH, spl[q].suf, sides[live[q]].r);
+            else add_w32(SH, spl[q].suf, sides[live[q]].r);
         }
         // content-addressed mid and suffix tables: one job per distinct key
         std::unordered_map<std::string, int> mid_job, suf_job;
@@ -909,33 +970,16 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         }
         if (std::getenv("GOLAY_PROFILE"))
             std::fprintf(stderr, "[golay] prep: %zu sides, %zu mid tables, %zu suffix tables, %zu fragment sets\n",
-                         live.size(), mid_rep.size(), suf_rep.size(), SH.frag_off.size() + SH.w32_off.size());
-        ph1.next("tables: mid/suffix jobs");
-        // offsets first, then every table is built in place on host threads
+                         live.size(), mid_rep.size(), suf_rep.size(), SH.frag_sets.size() + SH.w32_sets.size());
         std::vector<int64_t> mid_off(mid_rep.size()), suf_off(suf_rep.size());
-        int64_t n_mid = 0, n_suf = 0;
         for (size_t i = 0; i < mid_rep.size(); ++i) {
-            mid_off[i] = n_mid;
-            n_mid += mid_size(spl[mid_rep[i]], SH.mt());
+            mid_off[i] = SH.n_mid;
+            SH.n_mid += mid_size(spl[mid_rep[i]], SH.mt());
         }
         for (size_t i = 0; i < suf_rep.size(); ++i) {
-            suf_off[i] = n_suf;
-            n_suf += suf_size(spl[suf_rep[i]], fp32);
+            suf_off[i] = SH.n_suf;
+            SH.n_suf += suf_size(spl[suf_rep[i]], fp32);
         }
-        SH.mid.resize(n_mid);
-        if (fp32) SH.ent.resize(n_suf);
-        else SH.sufb.resize(n_suf);
-        parallel(mid_rep.size() + suf_rep.size(), [&](size_t i) {
-            if (i < mid_rep.size()) {
-                const size_t q = mid_rep[i];
-                build_mid(SH, sides[live[q]], spl[q].mid, SH.mid.data() + mid_off[i]);
-            } else {
-                const size_t k = i - mid_rep.size(), q = suf_rep[k];
-                build_suf(sides[live[q]], spl[q].suf, fp32, fp32 ? nullptr : SH.sufb.data() + suf_off[k],
-                          fp32 ? SH.ent.data() + suf_off[k] : nullptr);
-            }
-        });
-        ph1.next("tables: sides");
         // table order: sides grouped by operand class (one kernel instantiation each)
         const int mtc = SH.mt();
         const bool halfc = (int)SH.chan.size() <= 16 * (mtc - 1) + 8;
@@ -982,14 +1026,88 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             U.classes.back().sid_hi = (int)x + 1;
             U.classes.back().item_hi = T.items;
         }
-        T.sides.resize(live.size());
-        T.radix.assign(n_radix, 0);
-        T.tup.assign(n_tup, 0);
-        T.pm1.assign(n_pm1, 0);
-        T.pos32.assign(n_pos, 0);
-        T.pos64.assign(n_pos, 0);
-        T.olist.assign(n_olist, 0);
-        T.fix32.assign(n_fix, 0.f);
+        T.nsides = (int64_t)live.size();
+        if (T.nsides == 0) return prep;
+        // one staging buffer, regions 256-byte aligned
+        size_t bytes[kStCount] = {};
+        bytes[kStSides] = T.nsides * sizeof(UncSide);
+        bytes[kStRadix] = n_radix * sizeof(int32_t);
+        bytes[kStTup] = n_tup;
+        bytes[kStPm1] = n_pm1 * sizeof(uint64_t);
+        bytes[kStC64] = SH.c64.size() * sizeof(double);
+        bytes[kStPos64] = n_pos * sizeof(int32_t);
+        bytes[kStPos32] = n_pos * sizeof(int32_t);
+        bytes[kStOlist] = n_olist * sizeof(int32_t);
+        bytes[kStFix32] = n_fix * sizeof(float);
+        bytes[kStC32] = SH.c32.size() * sizeof(float);
+        bytes[kStMid] = SH.n_mid * sizeof(float);
+        bytes[kStSufb] = fp32 ? 0 : SH.n_suf * sizeof(uint4);
+        bytes[kStFrags] = SH.n_frags * sizeof(uint4);
+        bytes[kStEnt] = fp32 ? SH.n_suf : 0;
+        bytes[kStW32] = SH.n_w32 * sizeof(float);
+        bytes[kStToff] = sizeof(int64_t);
+        U.stage_off.assign(kStCount, 0);
+        size_t total = 0;
+        for (int i = 0; i < kStCount; ++i) {
+            U.stage_off[i] = total;
+            total += (bytes[i] + 255) & ~(size_t)255;
+        }
+        U.stage_bytes = total;
+        if (std::getenv("GOLAY_PROFILE"))
+            std::fprintf(stderr, "[golay] staging: %.2f MB (suffix operands %.2f MB, mid %.2f MB, sides %.2f MB)\n",
+                         total / 1e6, (bytes[kStSufb] + bytes[kStEnt]) / 1e6, bytes[kStMid] / 1e6,
+                         bytes[kStSides] / 1e6);
+        if (pinned) {
+            U.stage = pinned->acquire(total);
+            U.stage_pool = pinned;
+            U.base = static_cast<char*>(U.stage->p);
+        } else {
+            U.host.reset(new char[total]);
+            U.base = U.host.get();
+        }
+        auto at = [&](int i) { return static_cast<void*>(U.base + U.stage_off[i]); };
+        T.sides = static_cast<UncSide*>(at(kStSides));
+        T.radix = static_cast<int32_t*>(at(kStRadix));
+        T.tup = static_cast<int8_t*>(at(kStTup));
+        T.pm1 = static_cast<uint64_t*>(at(kStPm1));
+        T.pos64 = static_cast<int32_t*>(at(kStPos64));
+        T.pos32 = static_cast<int32_t*>(at(kStPos32));
+        T.olist = static_cast<int32_t*>(at(kStOlist));
+        T.fix32 = static_cast<float*>(at(kStFix32));
+        SH.mid = static_cast<float*>(at(kStMid));
+        SH.sufb = static_cast<uint4*>(at(kStSufb));
+        SH.frags = static_cast<uint4*>(at(kStFrags));
+        SH.ent = static_cast<int8_t*>(at(kStEnt));
+        SH.w32 = static_cast<float*>(at(kStW32));
+        std::memcpy(at(kStC64), SH.c64.data(), bytes[kStC64]);
+        std::memcpy(at(kStC32), SH.c32.data(), bytes[kStC32]);
+        std::memcpy(at(kStToff), &U.trig_off, sizeof(int64_t));
+        // rows that add_side leaves unwritten: tuples and masks d >= R[j] (never
+        // read) and the fixed-sum padding channels (read as 0)
+        std::memset(T.tup, 0, bytes[kStTup]);
+        std::memset(T.pm1, 0, bytes[kStPm1]);
+        std::memset(T.fix32, 0, bytes[kStFix32]);
+        ph1.next("tables: fill");
+        // every table built in place on host threads: weight sets, mid and suffix
+        // tables, then the per-side rows (which need the sets' BF16 residuals)
+        const size_t nset = fp32 ? SH.w32_sets.size() : SH.frag_sets.size();
+        std::vector<double> set_rho(nset, 0.0);
+        const int r0 = sides[live[0]].r, L0 = sides[live[0]].L;
+        parallel(nset + mid_rep.size() + suf_rep.size(), [&](size_t i) {
+            if (i < nset) {
+                if (fp32) build_w32(SH, SH.w32_sets[i], L0, r0, N, cs, sn);
+                else set_rho[i] = build_frags(SH, SH.frag_sets[i], L0, r0, N, cs, sn);
+            } else if (i < nset + mid_rep.size()) {
+                const size_t k = i - nset, q = mid_rep[k];
+                build_mid(SH, sides[live[q]], spl[q].mid, SH.mid + mid_off[k]);
+            } else {
+                const size_t k = i - nset - mid_rep.size(), q = suf_rep[k];
+                build_suf(sides[live[q]], spl[q].suf, fp32, fp32 ? nullptr : SH.sufb + suf_off[k],
+                          fp32 ? SH.ent + suf_off[k] : nullptr);
+            }
+        });
+        if (!fp32)
+            for (size_t i = 0; i < nset; ++i) SH.rho[SH.frag_sets[i]] = set_rho[i];
         parallel(live.size(), [&](size_t q) {
             add_side(T, slot[q], SH, sides[live[q]], spl[q], rmax, order, paf, group[live[q]], fp32,
                      mid_off[side_mid[q]], suf_off[side_suf[q]]);
@@ -1006,20 +1124,28 @@ void batch_header(const UncPrep& U, UncBatch& B) {
     B.trig_off = U.trig_off;
     B.bound = U.bound;
     B.side_of = U.side_of;
-    B.nsides_table = (int)U.T.sides.size();
+    B.nsides_table = (int)U.T.nsides;
 }
 
 // Survivor and band list capacities for a batch of `leaves` leaves.
 int64_t surv_cap(uint64_t leaves) { return (int64_t)std::min<uint64_t>(leaves * 3 / 5 + (1 << 20), (uint64_t)3 << 30); }
 int64_t band_cap(uint64_t leaves) { return (int64_t)std::min<uint64_t>(leaves / 64 + (1 << 16), (uint64_t)1 << 28); }
 
-// Uploads the staged tables into the device buffer `name` (one copy on the
-// context's stream), points B's table fields there and returns the filter
-// kernels' view of them.
-UncTabs upload_tables(gp_ctx* ctx, UncPrep& U, const std::string& name, UncBatch& B, const int64_t** d_toff) {
+// Uploads the staged tables into the device buffer `name` (one copy: on the
+// context's stream, or, with `uploaded`, on the copy stream, which the
+// context's stream then waits for, so that the copy overlaps earlier kernels),
+// points B's table fields there and returns the filter kernels' view of them.
+UncTabs upload_tables(gp_ctx* ctx, UncPrep& U, const std::string& name, UncBatch& B, const int64_t** d_toff,
+                      hipEvent_t uploaded = nullptr) {
     unc_stage(U, ctx->pinned);
     char* base = static_cast<char*>(ctx->buf(name).ensure(U.stage_bytes));
-    GP_HIP(hipMemcpyAsync(base, U.stage->p, U.stage_bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (uploaded) {
+        GP_HIP(hipMemcpyAsync(base, U.base, U.stage_bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+        GP_HIP(hipEventRecord(uploaded, ctx->copy_stream));
+        GP_HIP(hipStreamWaitEvent(ctx->stream, uploaded, 0));
+    } else {
+        GP_HIP(hipMemcpyAsync(base, U.base, U.stage_bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
     auto at = [&](int i) { return static_cast<void*>(base + U.stage_off[i]); };
     B.d_sides = static_cast<const UncSide*>(at(0));
     B.d_radix = static_cast<const int32_t*>(at(1));
@@ -1054,7 +1180,7 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
     } else {
         const int64_t waves = std::min<int64_t>(U.T.items, (int64_t)ctx->num_cus * 8);
         const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-        hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, tb, (int)U.T.sides.size(), U.T.items, ctl, d_surv,
+        hipLaunchKernelGGL(k_unc_filter_fp32, grid, dim3(256), 0, st, tb, (int)U.T.nsides, U.T.items, ctl, d_surv,
                            scap, d_band, bcap);
     }
     GP_HIP(hipGetLastError());
@@ -1086,7 +1212,7 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     hipStream_t st = ctx->stream;
     batch_header(U, B);
     add_dead_stats(stats, U);
-    if (U.T.sides.empty()) return B;
+    if (U.T.nsides == 0) return B;
     Phase ph2("unc_filter: upload");
     const int64_t* d_toff = nullptr;
     const UncTabs tb = upload_tables(ctx, U, "u_stage", B, &d_toff);
@@ -1108,7 +1234,7 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         GP_HIP(hipEventElapsedTime(&ms_filter, ctx->ev0, ctx->ev1));
         if ((int64_t)h.band > bcap) { bcap = (int64_t)h.band + 1024; continue; }
         if (h.band) {
-            hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup,
+            hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
                                ctx->tables.d_trig64.as<double>(), d_toff, d_band, ctl, d_surv, scap, (int64_t)h.band,
                                nullptr, (int64_t)0);
             GP_HIP(hipGetLastError());
@@ -1148,12 +1274,12 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     UncPrep& U = *F->prep;
     UncBatch& B = F->B;
     batch_header(U, B);
-    if (U.T.sides.empty()) return F;
+    if (U.T.nsides == 0) return F;
     hipStream_t st = ctx->stream;
     hipEvent_t* ev = ctx->slot_ev[slot];
     const std::string sfx = "#" + std::to_string(slot);
     const int64_t* d_toff = nullptr;
-    const UncTabs tb = upload_tables(ctx, U, "u_stage" + sfx, B, &d_toff);
+    const UncTabs tb = upload_tables(ctx, U, "u_stage" + sfx, B, &d_toff, ev[4]);
     F->scap = surv_cap(U.total_leaves);
     F->bcap = band_cap(U.total_leaves);
     F->ctl = ctx->dev<UncCtl>("u_ctl" + sfx, 1);
@@ -1164,13 +1290,14 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     GP_HIP(hipEventRecord(ev[0], st));
     launch_filters(ctx, U, tb, F->ctl, d_surv, F->scap, d_band, F->bcap);
     GP_HIP(hipEventRecord(ev[1], st));
-    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup,
+    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
                        ctx->tables.d_trig64.as<double>(), d_toff, d_band, F->ctl, d_surv, F->scap, (int64_t)0,
                        &F->ctl->band, F->bcap);
     GP_HIP(hipGetLastError());
     GP_HIP(hipEventRecord(ev[2], st));
     // join over at most rcap survivors (the batch is joined again synchronously
-    // when more survive); the table stays at most half full
+    // when more survive); the part of the table in use is sized on the device
+    // from the B rows counted by k_unc_rows and stays at most half full
     F->rcap = std::min<int64_t>(F->scap, (int64_t)(U.total_leaves / 32) + (1 << 20));
     uint64_t T = 1;
     while (T < (uint64_t)F->rcap * 2) T <<= 1;
@@ -1187,21 +1314,22 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     F->k = ctx->dev<uint8_t>("u_ek" + sfx, (size_t)2 * F->ecap * B.K);
     F->grp = ctx->dev<int32_t>("u_egrp" + sfx, (size_t)F->ecap);
     F->ties = ctx->dev<unsigned int>("u_eties" + sfx, 1);
-    GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
     GP_HIP(hipMemsetAsync(F->jctl, 0, sizeof(JoinCtl2), st));
     GP_HIP(hipMemsetAsync(F->ties, 0, 4, st));
     const unsigned long long* nsurv = &F->ctl->surv;
     const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, d_surv,
-                       (int64_t)0, paf_target, q_target, hashes, nsurv, F->rcap);
+                       (int64_t)0, paf_target, q_target, hashes, nsurv, F->rcap, F->jctl);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ht_fill, dim3(ctx->num_cus * 16), dim3(256), 0, st, table, T - 1, F->jctl);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
-                       nsurv, F->rcap);
+                       nsurv, F->rcap, F->jctl);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv, hashes,
-                       (int64_t)0, table, T - 1, paf_target, q_target, oa, ob, F->ecap, F->jctl, nsurv, F->rcap);
+                       (int64_t)0, table, T - 1, paf_target, q_target, oa, ob, F->ecap, F->jctl, nsurv, F->rcap, true);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_emit_pairs, dim3(ctx->num_cus * 2), dim3(128), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
+    hipLaunchKernelGGL(k_emit_pairs, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
                        oa, ob, &F->jctl->out, F->ecap, ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, F->w,
                        F->k, F->grp, F->ties);
     GP_HIP(hipGetLastError());
@@ -1215,7 +1343,7 @@ UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E,
     E = UncEmitted{};
     E.N = B.N;
     E.K = B.K;
-    if (U.T.sides.empty()) {
+    if (U.T.nsides == 0) {
         add_dead_stats(stats, U);
         return B;
     }
@@ -1274,7 +1402,7 @@ UncBatch unc_filter(gp_ctx* ctx, std::span<const SideHost> sides, std::span<cons
                     gp_stats* stats) {
     if (sides.empty()) return UncBatch{};
     const UncTrig trig = unc_trig(ctx, sides[0].N);
-    auto prep = unc_prepare(sides, group, order, paf, ctx->psd_mode == GP_PSD_FP32, trig);
+    auto prep = unc_prepare(sides, group, order, paf, ctx->psd_mode == GP_PSD_FP32, trig, &ctx->pinned);
     return unc_launch(ctx, *prep, stats);
 }
 
@@ -1296,10 +1424,10 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
     const unsigned grid = (unsigned)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
-                       paf_target, q_target, hashes, nullptr, (int64_t)0);
+                       paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, nullptr,
-                       (int64_t)0);
+                       (int64_t)0, nullptr);
     GP_HIP(hipGetLastError());
     int64_t cap = 1 << 16;
     JoinCtl2 h{};
@@ -1308,7 +1436,7 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
         uint32_t* ob = ctx->dev<uint32_t>("u_ob", (size_t)cap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
         hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_surv, hashes,
-                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl, nullptr, (int64_t)0);
+                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl, nullptr, (int64_t)0, false);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
@@ -1327,7 +1455,7 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
         int32_t* grp = ctx->dev<int32_t>("u_egrp", (size_t)2 * ne);
         unsigned int* ties = ctx->dev<unsigned int>("u_eties", 1);
         GP_HIP(hipMemsetAsync(ties, 0, 4, st));
-        hipLaunchKernelGGL(k_emit_rows, dim3((unsigned)std::min<int64_t>((2 * ne + 127) / 128, 4096)), dim3(128), 0, st,
+        hipLaunchKernelGGL(k_emit_rows, dim3((unsigned)std::min<int64_t>(ne, 4096)), dim3(64), 0, st,
                            B.d_sides, B.d_radix, B.d_tup, B.d_surv, rows, 2 * ne,
                            ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, w, k, grp, ties);
         GP_HIP(hipGetLastError());
