@@ -36,7 +36,8 @@ int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new
         GP_HIP(hipSetDevice(ctx->device));
         if (!orig || len <= 0 || compress <= 1 || new_compress <= 0 || compress % new_compress || !out)
             throw ArgError("gp_uncompress_side: bad arguments");
-        SideHost S = build_side(orig, len, compress, new_compress, seqflag);
+        const auto tab = make_splits(compress, new_compress);
+        const SideHost S = build_side(*tab, orig, len, seqflag);
         for (int i = 0; i < len; ++i) orig[i] = (int8_t)S.orig[i];  // the reference rotates in place
         const int order = len * compress;
         const int g0 = 0;
@@ -89,9 +90,12 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         // budget; the host tables of batch i + 2 are prepared on another thread
         // while the device runs batch i (the first batch is a quarter of the
         // budget, so that the device starts while the host prepares the second).
-        std::vector<SideHost> all((size_t)(2 * P));
-        std::vector<int> group(all.size());
-        for (size_t i = 0; i < all.size(); ++i) group[i] = (int)(i / 2);
+        static_assert(std::is_trivially_copyable_v<SideHost> && std::is_trivially_destructible_v<SideHost>);
+        if (ctx->unc_side_buf.size() < (size_t)(2 * P) * sizeof(SideHost))
+            ctx->unc_side_buf.resize((size_t)(2 * P) * sizeof(SideHost));
+        SideHost* const all = reinterpret_cast<SideHost*>(ctx->unc_side_buf.data());
+        std::vector<int> group((size_t)(2 * P));
+        for (size_t i = 0; i < group.size(); ++i) group[i] = (int)(i / 2);
         uint64_t leaf_budget = kLeafBudget;
         if (const char* e = std::getenv("GOLAY_UNC_BATCH_LEAVES"))  // smaller batches (tests)
             leaf_budget = std::max<uint64_t>(4, std::strtoull(e, nullptr, 10));
@@ -99,7 +103,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         if (P > 0) trig = unc_trig(ctx, N);
         const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
         auto prepare = [&, L](int64_t q0, int64_t q1) {
-            const std::span<const SideHost> sides(all.data() + 2 * q0, (size_t)(2 * (q1 - q0)));
+            const std::span<const SideHost> sides(all + 2 * q0, (size_t)(2 * (q1 - q0)));
             const std::span<const int> grp(group.data(), sides.size());
             GP_HIP(hipSetDevice(ctx->device));  // the pinned staging buffer
             return unc_prepare(sides, grp, order, paf, fp32, trig, &ctx->pinned);
@@ -108,8 +112,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         auto build_range = [&](int64_t q0, int64_t q1) {
             pool().parallel_for((size_t)(q1 - q0), [&](size_t i) {
                 const int64_t q = q0 + (int64_t)i;
-                all[2 * q] = build_side(tab, &inp.a[q * L], L, 0);
-                all[2 * q + 1] = build_side(tab, &inp.b[q * L], L, 1);
+                all[2 * q] = build_side(*tab, &inp.a[q * L], L, 0);
+                all[2 * q + 1] = build_side(*tab, &inp.b[q * L], L, 1);
             });
         };
         auto pair_leaves = [&](int64_t q) { return all[2 * q].leaves + all[2 * q + 1].leaves; };
@@ -161,6 +165,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         uint64_t ord_ties = 0, ord_pairs = 0;
         auto order_batch = [&, N](UncEmitted& E) {
             const int K = N / 2;
+            Phase ph("order: lines");
             std::string ta, tb;
             std::vector<size_t> oa, ob;
             build_lines(E.count, [&](std::string& t, int64_t i) {
@@ -172,6 +177,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             const LineSet LA{&ta, &oa, 3}, LB{&tb, &ob, 3};
             std::vector<uint64_t> ha, la;
             line_keys(LA, E.count, ha, la);
+            ph.next("order: sort");
             std::vector<int64_t> ord;
             for (int64_t i = 0; i < E.count; ++i) {
                 // key texts must agree (they do unless an FP64 PSD sits on a rounding tie)
@@ -188,6 +194,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 const std::string_view ax = LA.line(x), ay = LA.line(y);
                 return ax != ay ? ax < ay : LB.line(x) < LB.line(y);
             });
+            ph.next("order: append");
             for (int64_t i : ord) {
                 result.a.insert(result.a.end(), &E.words[i * N], &E.words[i * N] + N);
                 result.b.insert(result.b.end(), &E.words[(E.count + i) * N], &E.words[(E.count + i) * N] + N);
@@ -231,7 +238,11 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             inflight = std::move(F);
         }
         if (inflight) collect(*inflight, batches.size() - 1);
+        Phase ph_end("uncompress: last order");
         if (ordered.valid()) ordered.get();
+        ph_end.next("uncompress: release");
+        inflight.reset();
+        ph_end.next("uncompress: export");
         local.tie_band += ord_ties;
         local.pairs += ord_pairs;
         int rc = GP_OK;

@@ -172,6 +172,7 @@ struct gp_ctx {
     hipStream_t copy_stream = nullptr;  // read-backs that must not wait for later work on `stream`
     hipEvent_t slot_ev[2][5] = {};      // per batch slot: filter start, filter end, band end, batch end, tables uploaded
     int64_t unc_ecap = 1 << 16;         // emitted-pair rows per in-flight batch (grows on overflow)
+    std::vector<unsigned char> unc_side_buf;  // storage of the uncompress sides (reused across calls)
     int psd_mode = GP_PSD_SPLIT_BF16;
     std::string err;
     gp::PsdTables tables;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "golay_gpu.h"
+#include "gp_common.hpp"
 #include "gp_host.hpp"
 
 struct gp_ctx;
@@ -22,23 +23,25 @@ namespace gp {
 struct SplitTable {
     int r = 0;
     std::map<int, int> letter_id;
+    int16_t id_of[256];  // letter + 128 -> letter id, -1 if not in the alphabet
     std::vector<int> nsplit, nfirst;
     std::vector<std::vector<int8_t>> split, first;  // [letter id][tuple * r + i]
 };
 std::shared_ptr<const SplitTable> make_splits(int compress, int new_compress);
 
 // One compressed sequence prepared for expansion (uncompression.cpp:85-142).
+// Plain data (no heap storage), so that many sides live in one reusable buffer;
+// positions j < L are valid.
 struct SideHost {
-    int L = 0, r = 0, N = 0, seqflag = 0;
-    std::vector<int> orig;                 // after the reference's rotation loop
-    std::vector<int32_t> R;                // |T_j|: splits of position j
-    std::vector<const int8_t*> tp;         // T_j: R[j] r-tuples (into tab)
-    std::shared_ptr<const SplitTable> tab;
-    uint64_t leaves = 0;                   // product of R[j]
+    int L, r, N, seqflag;
+    int8_t orig[kMaxN];          // after the reference's rotation loop
+    int32_t R[kMaxN];            // |T_j|: splits of position j
+    const int8_t* tp[kMaxN];     // T_j: R[j] r-tuples (into tab)
+    const SplitTable* tab;       // outlives the side (held by the caller)
+    uint64_t leaves;             // product of R[j]
     const int8_t* tuple(int j, int d) const { return tp[j] + (size_t)d * r; }
 };
-SideHost build_side(const std::shared_ptr<const SplitTable>& tab, const int8_t* orig, int L, int seqflag);
-SideHost build_side(const int8_t* orig, int L, int compress, int new_compress, int seqflag);
+SideHost build_side(const SplitTable& tab, const int8_t* orig, int L, int seqflag);
 
 struct UncSide;
 

@@ -29,7 +29,7 @@ struct LineSet {
 // Builds m lines into one buffer on host threads: append(str, i) appends line i.
 template <class F>
 void build_lines(int64_t m, F append, std::string& text, std::vector<size_t>& off, int threads = 16) {
-    const int nt = std::max(1, std::min<int>(threads, (int)(m / 2048)));
+    const int nt = std::max(1, std::min<int>(threads, (int)(m / 256)));
     std::vector<std::string> part(nt);
     std::vector<std::vector<size_t>> poff(nt);
     pool().parallel_for(nt, [&](size_t t) {

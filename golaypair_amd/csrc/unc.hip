@@ -875,7 +875,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
     if (bound < 0 || bound > 255) throw ArgError("uncompress: keys need 0 <= 2*order - paf <= 255");
     int rmax = 1;
     for (const auto& s : sides)
-        for (int R : s.R) rmax = std::max(rmax, R);
+        for (int j = 0; j < s.L; ++j) rmax = std::max(rmax, (int)s.R[j]);
     U.N = N;
     U.bound = bound;
     U.rmax = rmax;
