@@ -110,15 +110,25 @@ int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
         ph.next("sort: sort");
         sort_lines(LineSet{&text, &off, 3}, idx);  // every line starts with "PSD"
         ph.next("sort: uniq + copy");
+        // uniq: a line equal to its predecessor is dropped; kept rows are
+        // numbered by a prefix sum over chunks and copied on host threads
+        std::vector<uint8_t> keep(m);
+        parallel_chunks(m, [&](int64_t r0, int64_t r1) {
+            for (int64_t r = r0; r < r1; ++r) keep[r] = r == 0 || line(idx[r]) != line(idx[r - 1]);
+        });
+        std::vector<int64_t> dst(m);
         int64_t w = 0;
         for (int64_t r = 0; r < m; ++r) {
-            if (w > 0 && line(idx[r]) == line(idx[w - 1])) continue;  // uniq
-            idx[w++] = idx[r];
+            dst[r] = w;
+            w += keep[r];
         }
-        for (int64_t r = 0; r < w; ++r) {
-            std::memcpy(c->words + r * n, &s.words[idx[r] * n], n);
-            std::memcpy(c->keys + r * K, &s.keys[idx[r] * K], K);
-        }
+        parallel_chunks(m, [&](int64_t r0, int64_t r1) {
+            for (int64_t r = r0; r < r1; ++r)
+                if (keep[r]) {
+                    std::memcpy(c->words + dst[r] * n, &s.words[idx[r] * n], n);
+                    std::memcpy(c->keys + dst[r] * K, &s.keys[idx[r] * K], K);
+                }
+        });
         c->count = w;
         return GP_OK;
     });

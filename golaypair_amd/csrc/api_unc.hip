@@ -88,8 +88,10 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         // Sides are built on host threads, the first batch's first, so that its
         // tables are prepared while the other sides are built.  Batches by leaf
         // budget; the host tables of batch i + 2 are prepared on another thread
-        // while the device runs batch i (the first batch is a quarter of the
-        // budget, so that the device starts while the host prepares the second).
+        // while the device runs batch i, one batch at a time (each preparation
+        // has every host thread); the first batch is a quarter of the budget and
+        // the second half of it, so that the device starts early and the host
+        // keeps ahead of it.
         static_assert(std::is_trivially_copyable_v<SideHost> && std::is_trivially_destructible_v<SideHost>);
         if (ctx->unc_side_buf.size() < (size_t)(2 * P) * sizeof(SideHost))
             ctx->unc_side_buf.resize((size_t)(2 * P) * sizeof(SideHost));
@@ -118,7 +120,15 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         };
         auto pair_leaves = [&](int64_t q) { return all[2 * q].leaves + all[2 * q + 1].leaves; };
         std::vector<std::pair<int64_t, int64_t>> batches;
-        std::future<std::shared_ptr<UncPrep>> next, after;
+        using PrepFuture = std::shared_future<std::shared_ptr<UncPrep>>;
+        PrepFuture next, after;
+        auto prepare_after = [&](PrepFuture prev, int64_t q0, int64_t q1) {
+            return std::async(std::launch::async, [&prepare, prev, q0, q1]() mutable {
+                       if (prev.valid()) prev.wait();
+                       prev = PrepFuture();  // the previous batch's tables are released with its own future
+                       return prepare(q0, q1);
+                   }).share();
+        };
         {
             Phase ph("uncompress: build sides");
             int64_t built = 0, q1 = 0;
@@ -140,7 +150,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
             }
             if (P > 0) {
                 batches.emplace_back(0, q1);
-                next = std::async(std::launch::async, prepare, (int64_t)0, q1);
+                next = prepare_after(PrepFuture(), 0, q1);
             }
             build_range(built, P);
             for (int64_t q0 = q1; q0 < P;) {
@@ -148,15 +158,16 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 int64_t e = q0;
                 while (e < P) {
                     const uint64_t add = pair_leaves(e);
-                    if (e > q0 && lv + add > leaf_budget) break;
+                    // the second batch is half the budget: its tables are ready sooner
+                    const uint64_t cap = batches.size() == 1 ? leaf_budget / 2 : leaf_budget;
+                    if (e > q0 && lv + add > cap) break;
                     lv += add;
                     ++e;
                 }
                 batches.emplace_back(q0, e);
                 q0 = e;
             }
-            if (batches.size() > 1)
-                after = std::async(std::launch::async, prepare, batches[1].first, batches[1].second);
+            if (batches.size() > 1) after = prepare_after(next, batches[1].first, batches[1].second);
         }
         // Emitted pairs of one batch in the reference's order (input pair, then A
         // line text, then B line text), appended to `result`.  Runs on a host
@@ -227,8 +238,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         for (size_t bi = 0; bi < batches.size(); ++bi) {
             std::shared_ptr<UncPrep> prep = next.get();
             next = std::move(after);
-            if (bi + 2 < batches.size())
-                after = std::async(std::launch::async, prepare, batches[bi + 2].first, batches[bi + 2].second);
+            after = PrepFuture();
+            if (bi + 2 < batches.size()) after = prepare_after(next, batches[bi + 2].first, batches[bi + 2].second);
             std::shared_ptr<UncInflight> F;
             {
                 Phase ph("uncompress: enqueue batch");

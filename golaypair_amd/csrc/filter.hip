@@ -245,30 +245,30 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     // Pairs with equal canonical keys form one class; its survivor is the pair
     // that comes first in (a, b) lexicographic order (golay.h:25-31).  Both orders
     // use packed big-endian keys (byte v + 128 per entry, a then b).
-    const int PW = (2 * n + 7) / 8;
-    std::vector<uint64_t> pk((size_t)m * PW, 0);
-    for (int64_t i = 0; i < m; ++i)
-        for (int j = 0; j < 2 * n; ++j) {
-            const int v = j < n ? in.a[i * n + j] : in.b[i * n + j - n];
-            pk[i * PW + j / 8] |= (uint64_t)(uint8_t)(v + 128) << (56 - 8 * (j % 8));
+    // one row per pair: the canonical key (2W words) then the pair key (PW
+    // words), so that (class, pair order) is one lexicographic word compare
+    const int PW = (2 * n + 7) / 8, CW = 2 * W, RW = CW + PW;
+    std::vector<uint64_t> rk((size_t)m * RW, 0);
+    parallel_chunks(m, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            uint64_t* r = &rk[i * RW];
+            for (int w = 0; w < CW; ++w) r[w] = canon[i * CW + w];
+            for (int j = 0; j < 2 * n; ++j) {
+                const int v = j < n ? in.a[i * n + j] : in.b[i * n + j - n];
+                r[CW + j / 8] |= (uint64_t)(uint8_t)(v + 128) << (56 - 8 * (j % 8));
+            }
         }
-    auto pair_less = [&](int64_t x, int64_t y) {
-        for (int w = 0; w < PW; ++w)
-            if (pk[x * PW + w] != pk[y * PW + w]) return pk[x * PW + w] < pk[y * PW + w];
+    });
+    auto words_less = [&](int64_t x, int64_t y, int w0, int w1) {
+        for (int w = w0; w < w1; ++w)
+            if (rk[x * RW + w] != rk[y * RW + w]) return rk[x * RW + w] < rk[y * RW + w];
         return false;
     };
-    auto canon_less = [&](int64_t x, int64_t y) {
-        for (int w = 0; w < 2 * W; ++w)
-            if (canon[x * 2 * W + w] != canon[y * 2 * W + w]) return canon[x * 2 * W + w] < canon[y * 2 * W + w];
-        return false;
-    };
+    auto pair_less = [&](int64_t x, int64_t y) { return words_less(x, y, CW, RW); };
+    auto canon_less = [&](int64_t x, int64_t y) { return words_less(x, y, 0, CW); };
     std::vector<int64_t> idx(m);
     std::iota(idx.begin(), idx.end(), 0);
-    parallel_sort(idx, [&](int64_t x, int64_t y) {
-        if (canon_less(x, y)) return true;
-        if (canon_less(y, x)) return false;
-        return pair_less(x, y);
-    });
+    parallel_sort(idx, [&](int64_t x, int64_t y) { return words_less(x, y, 0, RW); });
     std::vector<int64_t> keep;
     for (int64_t r = 0; r < m; ++r)
         if (r == 0 || canon_less(idx[r - 1], idx[r])) keep.push_back(idx[r]);

@@ -22,6 +22,7 @@
 
 #include "gp_internal.hpp"
 #include "gp_host.hpp"
+#include "linesort.hpp"
 #include "psd_core.hpp"
 
 namespace gp {
@@ -463,14 +464,16 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
                 int64_t i;
             };
             std::vector<Rec> rec(idx.size());
-            for (size_t x = 0; x < idx.size(); ++x) {
-                Rec& R = rec[x];
-                for (int w = 0; w < KW; ++w) R.k[w] = 0;
-                R.i = idx[x];
-                for (int j = 0; j < n; ++j)
-                    R.k[j / 8] |= (uint64_t)(uint8_t)(words_h[R.i * n + j] + 128) << (56 - 8 * (j % 8));
-            }
-            std::sort(rec.begin(), rec.end(), [](const Rec& x, const Rec& y) {
+            parallel_chunks((int64_t)idx.size(), [&](int64_t x0, int64_t x1) {
+                for (int64_t x = x0; x < x1; ++x) {
+                    Rec& R = rec[x];
+                    for (int w = 0; w < KW; ++w) R.k[w] = 0;
+                    R.i = idx[x];
+                    for (int j = 0; j < n; ++j)
+                        R.k[j / 8] |= (uint64_t)(uint8_t)(words_h[R.i * n + j] + 128) << (56 - 8 * (j % 8));
+                }
+            });
+            parallel_sort(rec, [](const Rec& x, const Rec& y) {
                 for (int w = 0; w < KW; ++w)
                     if (x.k[w] != y.k[w]) return x.k[w] < y.k[w];
                 return false;
@@ -489,10 +492,12 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         cs.nkey = K;
         cs.words.resize(idx.size() * n);
         cs.keys.resize(idx.size() * K);
-        for (size_t r = 0; r < idx.size(); ++r) {
-            std::memcpy(&cs.words[r * n], &words_h[idx[r] * n], n);
-            std::memcpy(&cs.keys[r * K], &keys_h[idx[r] * K], K);
-        }
+        parallel_chunks((int64_t)idx.size(), [&](int64_t r0, int64_t r1) {
+            for (int64_t r = r0; r < r1; ++r) {
+                std::memcpy(&cs.words[r * n], &words_h[idx[r] * n], n);
+                std::memcpy(&cs.keys[r * K], &keys_h[idx[r] * K], K);
+            }
+        });
     };
     emit(ia, out.a);
     emit(ib, out.b);

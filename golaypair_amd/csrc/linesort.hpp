@@ -53,9 +53,16 @@ void build_lines(int64_t m, F append, std::string& text, std::vector<size_t>& of
     off.push_back(text.size());
 }
 
+// Runs f(a, b) over [0, m) split into chunks of at least `grain` on host threads.
+template <class F>
+void parallel_chunks(int64_t m, F f, int64_t grain = 4096, int threads = 16) {
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads, m / grain));
+    pool().parallel_for(nt, [&](size_t t) { f(m * (int64_t)t / nt, m * (int64_t)(t + 1) / nt); });
+}
+
 // Sorts idx with `less` on host threads: chunks sorted concurrently, then merged pairwise.
-template <class Less>
-void parallel_sort(std::vector<int64_t>& idx, const Less& less, int threads = 16) {
+template <class T, class Less>
+void parallel_sort(std::vector<T>& idx, const Less& less, int threads = 16) {
     const int64_t m = (int64_t)idx.size();
     const int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
     std::vector<int64_t> cut(nt + 1);
@@ -74,7 +81,8 @@ void parallel_sort(std::vector<int64_t>& idx, const Less& less, int threads = 16
 inline void line_keys(const LineSet& L, int64_t m, std::vector<uint64_t>& hi, std::vector<uint64_t>& lo) {
     hi.assign(m, 0);
     lo.assign(m, 0);
-    for (int64_t i = 0; i < m; ++i) {
+    parallel_chunks(m, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
         const std::string_view s = L.line(i);
         uint64_t h = 0, l = 0;
         for (size_t b = 0; b < 16; ++b) {
@@ -86,6 +94,7 @@ inline void line_keys(const LineSet& L, int64_t m, std::vector<uint64_t>& hi, st
         hi[i] = h;
         lo[i] = l;
     }
+    });
 }
 
 // idx: a permutation of [0, m); sorted into line order.
