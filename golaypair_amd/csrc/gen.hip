@@ -233,14 +233,22 @@ __global__ void k_gen_unpack(const u128* __restrict__ surv_w, int64_t m, int n, 
 
 // Accepted rows, compacted: word (n bytes), FP64 key (K bytes: A plain rint(PSD_k),
 // B inverted bound - rint(PSD_k)) and side, each at its own slot (unordered).
-__global__ void k_gen_emit(const int8_t* __restrict__ words, int stride, int64_t m, int n,
-                           const double* __restrict__ trig, const uint8_t* __restrict__ decision,
-                           const uint8_t* __restrict__ side, int bound, unsigned long long* __restrict__ count,
-                           int8_t* __restrict__ ow, uint8_t* __restrict__ okeys, uint8_t* __restrict__ oside,
-                           unsigned int* __restrict__ ties) {
+// Blocks of 256; the twiddles and each thread's word are staged in LDS, and
+// PSD_k is summed in the order of fp64_psd.
+__global__ void __launch_bounds__(256) k_gen_emit(const int8_t* __restrict__ words, int stride, int64_t m, int n,
+                                                  const double* __restrict__ trig, const uint8_t* __restrict__ decision,
+                                                  const uint8_t* __restrict__ side, int bound,
+                                                  unsigned long long* __restrict__ count, int8_t* __restrict__ ow,
+                                                  uint8_t* __restrict__ okeys, uint8_t* __restrict__ oside,
+                                                  unsigned int* __restrict__ ties) {
+    __shared__ double tl[2 * kMaxN];
+    __shared__ int8_t wl[256][kMaxN];
+    for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) tl[t] = trig[t];
+    __syncthreads();
     const int K = n >> 1;
-    double psd[kMaxK];
     const int lane = threadIdx.x & 63;
+    int8_t* w = wl[threadIdx.x];
+    unsigned int t = 0;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = i0 + threadIdx.x;
         const bool acc = i < m && decision[i] == kAccept;
@@ -252,19 +260,20 @@ __global__ void k_gen_emit(const int8_t* __restrict__ words, int stride, int64_t
         base = __shfl(base, 0);
         if (!acc) continue;
         const unsigned long long slot = base + __popcll(bal & ((1ull << lane) - 1ull));
-        const int8_t* w = words + i * stride;
-        fp64_psd(w, n, trig, trig + n, psd);
-        unsigned int t = 0;
-        for (int k = 0; k < K; ++k) {
-            const double f = psd[k] - floor(psd[k]);
-            if (fabs(f - 0.5) < 1e-9) ++t;
-            const int v = (int)rint(psd[k]);
-            okeys[slot * K + k] = (uint8_t)(side[i] ? bound - v : v);
+        for (int j = 0; j < n; ++j) {
+            w[j] = words[i * stride + j];
+            ow[slot * n + j] = w[j];
         }
-        for (int j = 0; j < n; ++j) ow[slot * n + j] = w[j];
+        for (int k = 1; k <= K; ++k) {
+            const double p = fp64_psd_k(w, n, tl, tl + n, k);
+            const double f = p - floor(p);
+            if (fabs(f - 0.5) < 1e-9) ++t;
+            const int v = (int)rint(p);
+            okeys[slot * K + k - 1] = (uint8_t)(side[i] ? bound - v : v);
+        }
         oside[slot] = side[i];
-        if (t) atomicAdd(ties, t);
     }
+    if (t) atomicAdd(ties, t);
 }
 
 // ---------------------------------------------------------------------------

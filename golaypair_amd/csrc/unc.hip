@@ -123,8 +123,10 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
     uint64_t mask = 0;
     for (int j = S.L - 1; j >= 0; --j) {
         const uint32_t R = (uint32_t)radix[S.radix_off + j];
-        int d;
-        if (leaf < 0x100000000ull) {
+        int d = 0;
+        if (R == 1) {
+            // single-split position: digit 0, index unchanged
+        } else if (leaf < 0x100000000ull) {
             const uint32_t l32 = (uint32_t)leaf;
             d = (int)(l32 % R);
             leaf = l32 / R;
@@ -145,6 +147,8 @@ struct JoinCtl2 {
     unsigned long long out;     // emitted pairs
     unsigned long long checks;  // full PAF checks
     unsigned long long nb;      // B rows (counted by k_unc_rows when `jctl` is given)
+    unsigned long long slots;   // table slots read by the probes
+    unsigned long long ncand;   // candidate pairs (equal hashes, same group)
 };
 
 // Hash table size in use: the host's `tmask` + 1, or, with `jctl`, the
@@ -221,54 +225,93 @@ __global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* _
     }
 }
 
-__global__ void k_ht_probe(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
-                           const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+// A rows probe the table of B rows: every B row of the same group with the
+// same PAF hash is a candidate pair (A row i, B row j), checked exactly by
+// k_pair_check.
+__global__ void k_ht_probe(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
                            const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
-                           uint64_t tmask, int paf_target, int q_target, uint32_t* __restrict__ out_a,
-                           uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl, const unsigned long long* dcount,
-                           int64_t dcap, bool device_mask) {
-    int8_t wa[kMaxN], wb[kMaxN];
+                           uint64_t tmask, uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int64_t ccap,
+                           JoinCtl2* ctl, const unsigned long long* dcount, int64_t dcap, bool device_mask) {
     m = list_count(dcount, m, dcap);
     tmask = table_mask(device_mask ? ctl : nullptr, tmask);
-    unsigned long long checks = 0;
+    unsigned long long nslots = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t ra = surv[i];
         const UncSide& SA = sides[ra >> 40];
         if (SA.role != 0) continue;
         const uint64_t h = hashes[i];
         uint64_t slot = (h >> 24) & tmask;
-        bool built = false;
         for (;;) {
             const unsigned long long e = table[slot];
+            ++nslots;
             if (e == kEmpty) break;
             if ((e >> 32) == (h & 0xFFFFFFFFull)) {
                 const uint32_t j = (uint32_t)(e & 0xFFFFFFFFull);
-                const uint64_t rb = surv[j];
-                const UncSide& SB = sides[rb >> 40];
-                if (hashes[j] == h && SB.group == SA.group) {
-                    if (!built) { decode_word(SA, ra & ((1ull << 40) - 1), radix, tup, nullptr, wa); built = true; }
-                    decode_word(SB, rb & ((1ull << 40) - 1), radix, tup, nullptr, wb);
-                    ++checks;
-                    const int N = SA.N;
-                    int sq = 0;
-                    for (int p = 0; p < N; ++p) sq += (int)wa[p] * wa[p] + (int)wb[p] * wb[p];
-                    bool ok = sq == q_target;
-                    for (int s = 1; ok && s <= N / 2; ++s) {
-                        int acc = 0, k = s;
-                        for (int p = 0; p < N; ++p) {
-                            acc += (int)wa[p] * wa[k] + (int)wb[p] * wb[k];
-                            if (++k == N) k = 0;
-                        }
-                        ok = acc == paf_target;
-                    }
-                    if (ok) {
-                        const unsigned long long o = atomicAdd(&ctl->out, 1ull);
-                        if ((int64_t)o < cap) { out_a[o] = (uint32_t)i; out_b[o] = j; }
-                    }
+                if (hashes[j] == h && sides[surv[j] >> 40].group == SA.group) {
+                    const unsigned long long o = atomicAdd(&ctl->ncand, 1ull);
+                    if ((int64_t)o < ccap) { cand_a[o] = (uint32_t)i; cand_b[o] = j; }
                 }
             }
             slot = (slot + 1) & tmask;
         }
+    }
+    wave_atomic_add(&ctl->slots, nslots);
+}
+
+// Exact check of the candidate pairs: sum a_p^2 + b_p^2 = q_target and
+// PAF_a(s) + PAF_b(s) = paf_target for s = 1..N/2.  Blocks of one wave, one
+// pair per half-wave: the two words are decoded into LDS, lane q checks
+// s = q + 1 (lane 0 also the squares); passing pairs are appended to out.
+__global__ void __launch_bounds__(64) k_pair_check(const UncSide* __restrict__ sides,
+                                                   const int32_t* __restrict__ radix,
+                                                   const int8_t* __restrict__ tup, const uint64_t* __restrict__ surv,
+                                                   const uint32_t* __restrict__ cand_a,
+                                                   const uint32_t* __restrict__ cand_b, int64_t ccap, int paf_target,
+                                                   int q_target, uint32_t* __restrict__ out_a,
+                                                   uint32_t* __restrict__ out_b, int64_t cap, JoinCtl2* ctl) {
+    __shared__ int8_t wl[2][2][kMaxN];  // [half][A, B]
+    const int64_t nc = list_count(&ctl->ncand, 0, ccap);
+    const int h = threadIdx.x >> 5, q = threadIdx.x & 31;
+    unsigned long long checks = 0;
+    for (int64_t i0 = 2 * (int64_t)blockIdx.x; i0 < nc; i0 += 2 * (int64_t)gridDim.x) {
+        const int64_t i = i0 + h;
+        const bool live = i < nc;
+        int N = 0;
+        if (live) {
+            const uint64_t rec = surv[q == 1 ? cand_b[i] : cand_a[i]];
+            const UncSide& S = sides[rec >> 40];
+            N = S.N;
+            if (q < 2) decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, wl[h][q]);
+        }
+        __syncthreads();
+        bool ok = true;
+        if (live) {
+            const int8_t* wa = wl[h][0];
+            const int8_t* wb = wl[h][1];
+            const int s = q + 1;
+            if (s <= N / 2) {
+                int acc = 0, k = s;
+                for (int p = 0; p < N; ++p) {
+                    acc += (int)wa[p] * wa[k] + (int)wb[p] * wb[k];
+                    if (++k == N) k = 0;
+                }
+                ok = acc == paf_target;
+            }
+            if (q == 0) {
+                int sq = 0;
+                for (int p = 0; p < N; ++p) sq += (int)wa[p] * wa[p] + (int)wb[p] * wb[p];
+                ok = ok && sq == q_target;
+            }
+        }
+        const uint64_t bad = __ballot(!ok);
+        if (live && q == 0) {
+            ++checks;
+            if (((bad >> (32 * h)) & 0xFFFFFFFFull) == 0) {
+                const unsigned long long o = atomicAdd(&ctl->out, 1ull);
+                if ((int64_t)o < cap) { out_a[o] = cand_a[i]; out_b[o] = cand_b[i]; }
+            }
+        }
+        __syncthreads();  // wl is rewritten by the next iteration
     }
     wave_atomic_add(&ctl->checks, checks);
 }
@@ -336,6 +379,56 @@ __global__ void __launch_bounds__(64) k_emit_rows(const UncSide* __restrict__ si
                                                   int32_t* __restrict__ groups, unsigned int* __restrict__ ties) {
     emit_rows_halfwave(
         m, m, [&](int64_t i) { return surv[rows[i]]; }, sides, radix, tup, trig, bound, words, keys, groups, ties);
+}
+
+// One suffix operand table of the split filter: the B fragments of every
+// combination of the suffix positions' tuples (mixed-radix order, last
+// position fastest), laid out [tile][block][lane] as uint4 of 8 BF16 (entry
+// e = 16 block + 8 (lane / 32) + i of combination 32 tile + lane % 32, entry
+// e = x r + i being element i of suffix position x's tuple).  `side` is a
+// table side with these positions and letters (its tuple rows are used).
+struct SufJob {
+    int64_t out_off;  // in uint4
+    int32_t side, npos, pos_off, pad;
+};
+
+__global__ void __launch_bounds__(256) k_build_suf(const SufJob* __restrict__ jobs, const int32_t* __restrict__ pos,
+                                                   const UncSide* __restrict__ sides,
+                                                   const int32_t* __restrict__ radix,
+                                                   const int8_t* __restrict__ tup, uint4* __restrict__ sufb) {
+    const SufJob J = jobs[blockIdx.x];
+    const UncSide& S = sides[J.side];
+    const int r = S.r, nent = J.npos * r, nbs = nent > 16 ? (nent + 15) / 16 : 1;
+    const int32_t* sp = pos + J.pos_off;
+    int64_t ps = 1;
+    for (int x = 0; x < J.npos; ++x) ps *= radix[S.radix_off + sp[x]];
+    const int64_t ntiles = (ps + 31) / 32;
+    uint4* out = sufb + J.out_off;
+    for (int64_t u = threadIdx.x; u < ntiles * nbs * 64; u += blockDim.x) {
+        const int lane = (int)(u & 63), blk = (int)((u >> 6) % nbs);
+        const int64_t tile = (u >> 6) / nbs;
+        const int64_t s0 = tile * 32 + (lane & 31);
+        uint32_t hv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (s0 < ps) {
+            int dg[kMaxN];
+            int64_t rest = s0;
+            for (int x = J.npos - 1; x >= 0; --x) {
+                const int R = radix[S.radix_off + sp[x]];
+                dg[x] = (int)(rest % R);
+                rest /= R;
+            }
+            for (int i8 = 0; i8 < 8; ++i8) {
+                const int e = 16 * blk + 8 * (lane >> 5) + i8;
+                if (e < nent) {
+                    const int x = e / r, i = e % r;
+                    const float v = (float)tup[S.tup_off + ((int64_t)sp[x] * S.rmax + dg[x]) * r + i];
+                    hv[i8] = __float_as_uint(v) >> 16;  // small integers are exact in BF16
+                }
+            }
+        }
+        out[u] = make_uint4(hv[0] | (hv[1] << 16), hv[2] | (hv[3] << 16), hv[4] | (hv[5] << 16),
+                            hv[6] | (hv[7] << 16));
+    }
 }
 
 // Records of one table side (unordered) -> compact list.
@@ -798,6 +891,7 @@ struct UncPrep {
     std::vector<UncClass> classes;
     uint64_t total_leaves = 0;
     uint64_t dead_leaves = 0;  // leaves of sides rejected by their fixed channels
+    int64_t n_suf_jobs = 0, n_sufb = 0;  // split path: suffix operand jobs and their size (uint4)
     // every table in one staging buffer (one host-to-device copy): pinned
     // (`stage`) or, when prepared without a pool, `host` until unc_stage
     PinnedBuf* stage = nullptr;
@@ -816,7 +910,7 @@ namespace {
 // Staging buffer regions, in upload order (sizes in bytes).
 enum StageRegion {
     kStSides, kStRadix, kStTup, kStPm1, kStC64, kStPos64, kStPos32, kStOlist, kStFix32, kStC32,
-    kStMid, kStSufb, kStFrags, kStEnt, kStW32, kStToff, kStCount
+    kStMid, kStSufJobs, kStSufPos, kStFrags, kStEnt, kStW32, kStToff, kStCount
 };
 
 }  // namespace
@@ -1041,7 +1135,12 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         bytes[kStFix32] = n_fix * sizeof(float);
         bytes[kStC32] = SH.c32.size() * sizeof(float);
         bytes[kStMid] = SH.n_mid * sizeof(float);
-        bytes[kStSufb] = fp32 ? 0 : SH.n_suf * sizeof(uint4);
+        // split path: suffix operands are built on the device from a job list
+        int64_t n_sufpos = 0;
+        if (!fp32)
+            for (size_t i = 0; i < suf_rep.size(); ++i) n_sufpos += (int64_t)spl[suf_rep[i]].suf.size();
+        bytes[kStSufJobs] = fp32 ? 0 : suf_rep.size() * sizeof(SufJob);
+        bytes[kStSufPos] = n_sufpos * sizeof(int32_t);
         bytes[kStFrags] = SH.n_frags * sizeof(uint4);
         bytes[kStEnt] = fp32 ? SH.n_suf : 0;
         bytes[kStW32] = SH.n_w32 * sizeof(float);
@@ -1055,7 +1154,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         U.stage_bytes = total;
         if (std::getenv("GOLAY_PROFILE"))
             std::fprintf(stderr, "[golay] staging: %.2f MB (suffix operands %.2f MB, mid %.2f MB, sides %.2f MB)\n",
-                         total / 1e6, (bytes[kStSufb] + bytes[kStEnt]) / 1e6, bytes[kStMid] / 1e6,
+                         total / 1e6, bytes[kStEnt] / 1e6, bytes[kStMid] / 1e6,
                          bytes[kStSides] / 1e6);
         if (pinned) {
             U.stage = pinned->acquire(total);
@@ -1075,7 +1174,18 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         T.olist = static_cast<int32_t*>(at(kStOlist));
         T.fix32 = static_cast<float*>(at(kStFix32));
         SH.mid = static_cast<float*>(at(kStMid));
-        SH.sufb = static_cast<uint4*>(at(kStSufb));
+        U.n_suf_jobs = fp32 ? 0 : (int64_t)suf_rep.size();
+        U.n_sufb = fp32 ? 0 : SH.n_suf;
+        if (!fp32) {
+            auto* jobs = static_cast<SufJob*>(at(kStSufJobs));
+            auto* spos = static_cast<int32_t*>(at(kStSufPos));
+            int64_t po = 0;
+            for (size_t i = 0; i < suf_rep.size(); ++i) {
+                const Split& X = spl[suf_rep[i]];
+                jobs[i] = SufJob{suf_off[i], slot[suf_rep[i]].sid, (int32_t)X.suf.size(), (int32_t)po, 0};
+                for (int j : X.suf) spos[po++] = j;
+            }
+        }
         SH.frags = static_cast<uint4*>(at(kStFrags));
         SH.ent = static_cast<int8_t*>(at(kStEnt));
         SH.w32 = static_cast<float*>(at(kStW32));
@@ -1093,7 +1203,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         const size_t nset = fp32 ? SH.w32_sets.size() : SH.frag_sets.size();
         std::vector<double> set_rho(nset, 0.0);
         const int r0 = sides[live[0]].r, L0 = sides[live[0]].L;
-        parallel(nset + mid_rep.size() + suf_rep.size(), [&](size_t i) {
+        auto fill_one = [&](size_t i) {
             if (i < nset) {
                 if (fp32) build_w32(SH, SH.w32_sets[i], L0, r0, N, cs, sn);
                 else set_rho[i] = build_frags(SH, SH.frag_sets[i], L0, r0, N, cs, sn);
@@ -1102,10 +1212,10 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 build_mid(SH, sides[live[q]], spl[q].mid, SH.mid + mid_off[k]);
             } else {
                 const size_t k = i - nset - mid_rep.size(), q = suf_rep[k];
-                build_suf(sides[live[q]], spl[q].suf, fp32, fp32 ? nullptr : SH.sufb + suf_off[k],
-                          fp32 ? SH.ent + suf_off[k] : nullptr);
+                build_suf(sides[live[q]], spl[q].suf, true, nullptr, SH.ent + suf_off[k]);
             }
-        });
+        };
+        parallel(nset + mid_rep.size() + (fp32 ? suf_rep.size() : 0), fill_one);
         if (!fp32)
             for (size_t i = 0; i < nset; ++i) SH.rho[SH.frag_sets[i]] = set_rho[i];
         parallel(live.size(), [&](size_t q) {
@@ -1147,24 +1257,33 @@ UncTabs upload_tables(gp_ctx* ctx, UncPrep& U, const std::string& name, UncBatch
         GP_HIP(hipMemcpyAsync(base, U.base, U.stage_bytes, hipMemcpyHostToDevice, ctx->stream));
     }
     auto at = [&](int i) { return static_cast<void*>(base + U.stage_off[i]); };
-    B.d_sides = static_cast<const UncSide*>(at(0));
-    B.d_radix = static_cast<const int32_t*>(at(1));
-    B.d_tup = static_cast<const int8_t*>(at(2));
-    B.d_pm1 = static_cast<const uint64_t*>(at(3));
-    B.d_c64 = static_cast<const double*>(at(4));
-    B.d_pos64 = static_cast<const int32_t*>(at(5));
-    *d_toff = static_cast<const int64_t*>(at(15));
+    B.d_sides = static_cast<const UncSide*>(at(kStSides));
+    B.d_radix = static_cast<const int32_t*>(at(kStRadix));
+    B.d_tup = static_cast<const int8_t*>(at(kStTup));
+    B.d_pm1 = static_cast<const uint64_t*>(at(kStPm1));
+    B.d_c64 = static_cast<const double*>(at(kStC64));
+    B.d_pos64 = static_cast<const int32_t*>(at(kStPos64));
+    *d_toff = static_cast<const int64_t*>(at(kStToff));
+    // split path: the suffix operands, built on the device from the job list
+    uint4* sufb = nullptr;
+    if (U.n_suf_jobs > 0) {
+        sufb = static_cast<uint4*>(ctx->buf(name + ".sufb").ensure((size_t)U.n_sufb * sizeof(uint4)));
+        hipLaunchKernelGGL(k_build_suf, dim3((unsigned)U.n_suf_jobs), dim3(256), 0, ctx->stream,
+                           static_cast<const SufJob*>(at(kStSufJobs)), static_cast<const int32_t*>(at(kStSufPos)),
+                           B.d_sides, B.d_radix, B.d_tup, sufb);
+        GP_HIP(hipGetLastError());
+    }
     return UncTabs{B.d_sides,
                    B.d_radix,
-                   static_cast<const int32_t*>(at(6)),
-                   static_cast<const int32_t*>(at(7)),
-                   static_cast<const float*>(at(9)),
-                   static_cast<const float*>(at(10)),
-                   static_cast<const float*>(at(8)),
-                   static_cast<const uint4*>(at(11)),
-                   static_cast<const uint4*>(at(12)),
-                   static_cast<const int8_t*>(at(13)),
-                   static_cast<const float*>(at(14))};
+                   static_cast<const int32_t*>(at(kStPos32)),
+                   static_cast<const int32_t*>(at(kStOlist)),
+                   static_cast<const float*>(at(kStC32)),
+                   static_cast<const float*>(at(kStMid)),
+                   static_cast<const float*>(at(kStFix32)),
+                   sufb,
+                   static_cast<const uint4*>(at(kStFrags)),
+                   static_cast<const int8_t*>(at(kStEnt)),
+                   static_cast<const float*>(at(kStW32))};
 }
 
 // The filter kernels of one batch on the context's stream.
@@ -1258,7 +1377,7 @@ struct UncInflight {
     std::shared_ptr<UncPrep> prep;  // keeps the staged tables for a synchronous rerun
     int slot = 0;
     UncBatch B;
-    int64_t scap = 0, bcap = 0, rcap = 0, ecap = 0;
+    int64_t scap = 0, bcap = 0, rcap = 0, ecap = 0, ccap = 0;
     UncCtl* ctl = nullptr;
     JoinCtl2* jctl = nullptr;
     unsigned int* ties = nullptr;
@@ -1308,6 +1427,9 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     F->ecap = ctx->unc_ecap;
     if (const char* e = std::getenv("GOLAY_UNC_EMIT_CAP"))  // fixed emitted-pair capacity (tests)
         F->ecap = std::max<int64_t>(1, std::strtoll(e, nullptr, 10));
+    F->ccap = 2 * F->ecap;
+    uint32_t* ca = ctx->dev<uint32_t>("u_ca" + sfx, (size_t)F->ccap);
+    uint32_t* cb = ctx->dev<uint32_t>("u_cb" + sfx, (size_t)F->ccap);
     uint32_t* oa = ctx->dev<uint32_t>("u_oa" + sfx, (size_t)F->ecap);
     uint32_t* ob = ctx->dev<uint32_t>("u_ob" + sfx, (size_t)F->ecap);
     F->w = ctx->dev<int8_t>("u_ew" + sfx, (size_t)2 * F->ecap * B.N);
@@ -1326,8 +1448,11 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
                        nsurv, F->rcap, F->jctl);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv, hashes,
-                       (int64_t)0, table, T - 1, paf_target, q_target, oa, ob, F->ecap, F->jctl, nsurv, F->rcap, true);
+    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
+                       ca, cb, F->ccap, F->jctl, nsurv, F->rcap, true);
+    GP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pair_check, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
+                       ca, cb, F->ccap, paf_target, q_target, oa, ob, F->ecap, F->jctl);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_emit_pairs, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
                        oa, ob, &F->jctl->out, F->ecap, ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, F->w,
@@ -1370,12 +1495,15 @@ UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E,
     add_dead_stats(stats, U);
     add_filter_stats(stats, U, h, ms_filter, ms_band);
     if (std::getenv("GOLAY_PROFILE"))
-        std::fprintf(stderr, "[golay] batch: %llu leaves, %llu survivors; device filter %.3f, band %.3f, join %.3f ms\n",
-                     (unsigned long long)U.total_leaves, (unsigned long long)h.surv, ms_filter, ms_band, ms_join);
+        std::fprintf(stderr,
+                     "[golay] batch: %llu leaves, %llu survivors (%llu B rows, %llu slots probed); device filter "
+                     "%.3f, band %.3f, join %.3f ms\n",
+                     (unsigned long long)U.total_leaves, (unsigned long long)h.surv, hj.nb, hj.slots, ms_filter,
+                     ms_band, ms_join);
     B.nsurv = (int64_t)h.surv;
-    if ((int64_t)h.surv > F.rcap || (int64_t)hj.out > F.ecap) {
+    if ((int64_t)h.surv > F.rcap || (int64_t)hj.ncand > F.ccap || (int64_t)hj.out > F.ecap) {
         // the join's lists were too small: join again with exact sizes
-        if ((int64_t)hj.out > F.ecap) ctx->unc_ecap = std::max<int64_t>(ctx->unc_ecap, (int64_t)hj.out * 2);
+        ctx->unc_ecap = std::max<int64_t>(ctx->unc_ecap, 2 * (int64_t)std::max<uint64_t>(hj.out, hj.ncand / 2));
         E = unc_join(ctx, B, paf_target, stats);
         return B;
     }
@@ -1429,19 +1557,32 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, nullptr,
                        (int64_t)0, nullptr);
     GP_HIP(hipGetLastError());
-    int64_t cap = 1 << 16;
+    // candidate pairs (the list is grown once if it overflows), then the exact
+    // check, whose output cannot exceed the candidates
+    int64_t ccap = 1 << 16;
     JoinCtl2 h{};
     for (int attempt = 0; attempt < 2; ++attempt) {
-        uint32_t* oa = ctx->dev<uint32_t>("u_oa", (size_t)cap);
-        uint32_t* ob = ctx->dev<uint32_t>("u_ob", (size_t)cap);
+        uint32_t* ca = ctx->dev<uint32_t>("u_ca", (size_t)ccap);
+        uint32_t* cb = ctx->dev<uint32_t>("u_cb", (size_t)ccap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
-        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_surv, hashes,
-                           m, table, T - 1, paf_target, q_target, oa, ob, cap, ctl, nullptr, (int64_t)0, false);
+        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, ca,
+                           cb, ccap, ctl, nullptr, (int64_t)0, false);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
-        if ((int64_t)h.out <= cap) break;
-        cap = (int64_t)h.out;
+        if ((int64_t)h.ncand <= ccap) break;
+        ccap = (int64_t)h.ncand;
+    }
+    const int64_t cap = std::max<int64_t>(1, (int64_t)h.ncand);
+    {
+        uint32_t* oa = ctx->dev<uint32_t>("u_oa", (size_t)cap);
+        uint32_t* ob = ctx->dev<uint32_t>("u_ob", (size_t)cap);
+        hipLaunchKernelGGL(k_pair_check, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
+                           B.d_surv, ctx->buf("u_ca").as<uint32_t>(), ctx->buf("u_cb").as<uint32_t>(), ccap,
+                           paf_target, q_target, oa, ob, cap, ctl);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
     }
     const int64_t ne = (int64_t)h.out;
     E.count = ne;
