@@ -482,10 +482,11 @@ struct Shared {
     std::vector<int32_t> off32, off64;
     std::vector<float> c32;
     std::vector<double> c64;
-    std::map<std::vector<int>, int64_t> frag_off;  // suffix positions -> offset into frags
-    std::map<std::vector<int>, double> rho;        // suffix positions -> max BF16 split residual
-    std::map<std::vector<int>, int64_t> w32_off;   // suffix positions -> offset into w32
-    std::vector<std::vector<int>> frag_sets, w32_sets;  // in offset order
+    // suffix weight sets (one per distinct suffix position set): positions,
+    // offset into frags (split path) or w32 (FP32 path), max BF16 split residual
+    std::vector<std::vector<int>> sets;
+    std::vector<int64_t> set_off;
+    std::vector<double> set_rho;
     int64_t n_frags = 0, n_w32 = 0, n_mid = 0, n_suf = 0;
     // views into the staging buffer
     uint4* frags = nullptr;  // MFMA A fragments per suffix position set (split path)
@@ -632,19 +633,28 @@ int64_t suf_size(const Split& X, bool fp32) {
 }
 
 // FP32 suffix DFT weights of the suffix positions, [nent][mt*32] (FP32 path).
-// Registers the suffix position set `sp` (offset and size in w32).
-void add_w32(Shared& SH, const std::vector<int>& sp, int r) {
-    auto [it, fresh] = SH.w32_off.emplace(sp, SH.n_w32);
-    if (!fresh) return;
-    SH.w32_sets.push_back(sp);
-    SH.n_w32 += (int64_t)sp.size() * r * SH.mt() * 32;
+// Registers a new suffix position set `sp`: its offset and size in w32 (FP32
+// path) or frags (split path); returns its index.
+int add_set(Shared& SH, const std::vector<int>& sp, int r, bool fp32) {
+    const int nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
+    SH.sets.push_back(sp);
+    SH.set_rho.push_back(0.0);
+    if (fp32) {
+        SH.set_off.push_back(SH.n_w32);
+        SH.n_w32 += (int64_t)nent * SH.mt() * 32;
+    } else {
+        SH.set_off.push_back(SH.n_frags);
+        SH.n_frags += (int64_t)SH.mt() * 2 * nbs * 64;
+    }
+    return (int)SH.sets.size() - 1;
 }
 
-// Writes the weights of a registered set.
-void build_w32(const Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+// Writes the FP32 weights of set `set`.
+void build_w32(const Shared& SH, int set, int L, int r, int N, const std::vector<double>& cs,
                const std::vector<double>& sn) {
+    const std::vector<int>& sp = SH.sets[set];
     const int mt = SH.mt(), nent = (int)sp.size() * r;
-    float* out = SH.w32 + SH.w32_off.at(sp);
+    float* out = SH.w32 + SH.set_off[set];
     for (int e = 0; e < nent; ++e) {
         const int j = sp[e / r], i = e % r, p = j + L * i;
         for (int t = 0; t < mt; ++t)
@@ -699,21 +709,13 @@ void add_blocks(Shared& SH, const SplitTable& X, int L, int N, const std::vector
         }
 }
 
-// Registers the suffix position set `sp` (offset and size in frags).
-void add_frags(Shared& SH, const std::vector<int>& sp, int r) {
-    auto [it, fresh] = SH.frag_off.emplace(sp, SH.n_frags);
-    if (!fresh) return;
-    SH.frag_sets.push_back(sp);
-    const int nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
-    SH.n_frags += (int64_t)SH.mt() * 2 * nbs * 64;
-}
-
-// Writes the fragments of a registered set; returns the largest BF16 split
+// Writes the MFMA A fragments of set `set`; returns the largest BF16 split
 // residual |w - hi - lo| of its weights.
-double build_frags(const Shared& SH, const std::vector<int>& sp, int L, int r, int N, const std::vector<double>& cs,
+double build_frags(const Shared& SH, int set, int L, int r, int N, const std::vector<double>& cs,
                    const std::vector<double>& sn) {
+    const std::vector<int>& sp = SH.sets[set];
     const int mt = SH.mt(), nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
-    uint4* out = SH.frags + SH.frag_off.at(sp);
+    uint4* out = SH.frags + SH.set_off[set];
     double rho = 0;
     for (int t = 0; t < mt; ++t)
         for (int st = 0; st < 2 * nbs; ++st)
@@ -767,7 +769,7 @@ bool side_pm1(const SideHost& S) {
 }
 
 void add_side(Tables& T, const SideSlot& Z, const Shared& SH, const SideHost& S, const Split& X, int rmax,
-              int order, int paf, int group, bool fp32, int64_t mid_off, int64_t suf_off) {
+              int order, int paf, int group, bool fp32, int64_t mid_off, int64_t suf_off, int set) {
     UncSide D{};
     D.L = S.L; D.r = S.r; D.N = S.N; D.K = S.N / 2; D.seqflag = S.seqflag; D.rmax = rmax;
     D.group = group;
@@ -825,12 +827,12 @@ void add_side(Tables& T, const SideSlot& Z, const Shared& SH, const SideHost& S,
     D.mid_off = mid_off;
     D.sufb_off = fp32 ? 0 : suf_off;
     D.ent_off = fp32 ? suf_off : 0;
-    D.w32_off = fp32 ? SH.w32_off.at(X.suf) : 0;
+    D.w32_off = fp32 ? SH.set_off[set] : 0;
     // A fragments (suffix DFT weights, BF16 hi and lo), shared per suffix position set
-    D.frag_off = fp32 ? 0 : SH.frag_off.at(X.suf);
-    const double rho = fp32 ? 0.0 : SH.rho.at(X.suf);
-    std::vector<char> in_suf(L, 0);
-    for (int j : X.suf) in_suf[j] = 1;
+    D.frag_off = fp32 ? 0 : SH.set_off[set];
+    const double rho = fp32 ? 0.0 : SH.set_rho[set];
+    bool in_suf[kMaxN] = {};
+    for (int j : X.suf) in_suf[j] = true;
     // error bound of the filter's Re/Im
     double s1_pre = 0, s1_suf = 0;
     for (int j = 0; j < L; ++j) {
@@ -1045,16 +1047,20 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             pm1[q] = side_pm1(S);
         });
         ph1.next("tables: layout");
-        // suffix weight sets (built below, in parallel)
-        for (size_t q = 0; q < live.size(); ++q) {
-            if (!fp32) add_frags(SH, spl[q].suf, sides[live[q]].r);
-            else add_w32(SH, spl[q].suf, sides[live[q]].r);
-        }
-        // content-addressed mid and suffix tables: one job per distinct key
-        std::unordered_map<std::string, int> mid_job, suf_job;
+        // content-addressed suffix weight sets (key: the suffix positions, the
+        // first half of the suffix key), mid and suffix tables (built below, in
+        // parallel): one per distinct key
+        std::unordered_map<std::string_view, int> set_of, mid_job, suf_job;
+        set_of.reserve(64);
+        mid_job.reserve(live.size());
+        suf_job.reserve(live.size());
         std::vector<size_t> mid_rep, suf_rep;  // representative side (index into live) per job
-        std::vector<int> side_mid(live.size()), side_suf(live.size());
+        std::vector<int> side_mid(live.size()), side_suf(live.size()), side_set(live.size());
         for (size_t q = 0; q < live.size(); ++q) {
+            const std::string_view kpos(ksuf[q].data(), ksuf[q].size() / 2);
+            auto it = set_of.find(kpos);
+            if (it == set_of.end()) it = set_of.emplace(kpos, add_set(SH, spl[q].suf, sides[live[q]].r, fp32)).first;
+            side_set[q] = it->second;
             auto im = mid_job.emplace(kmid[q], (int)mid_rep.size());
             if (im.second) mid_rep.push_back(q);
             side_mid[q] = im.first->second;
@@ -1064,7 +1070,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         }
         if (std::getenv("GOLAY_PROFILE"))
             std::fprintf(stderr, "[golay] prep: %zu sides, %zu mid tables, %zu suffix tables, %zu fragment sets\n",
-                         live.size(), mid_rep.size(), suf_rep.size(), SH.frag_sets.size() + SH.w32_sets.size());
+                         live.size(), mid_rep.size(), suf_rep.size(), SH.sets.size());
         std::vector<int64_t> mid_off(mid_rep.size()), suf_off(suf_rep.size());
         for (size_t i = 0; i < mid_rep.size(); ++i) {
             mid_off[i] = SH.n_mid;
@@ -1200,13 +1206,12 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         ph1.next("tables: fill");
         // every table built in place on host threads: weight sets, mid and suffix
         // tables, then the per-side rows (which need the sets' BF16 residuals)
-        const size_t nset = fp32 ? SH.w32_sets.size() : SH.frag_sets.size();
-        std::vector<double> set_rho(nset, 0.0);
+        const size_t nset = SH.sets.size();
         const int r0 = sides[live[0]].r, L0 = sides[live[0]].L;
         auto fill_one = [&](size_t i) {
             if (i < nset) {
-                if (fp32) build_w32(SH, SH.w32_sets[i], L0, r0, N, cs, sn);
-                else set_rho[i] = build_frags(SH, SH.frag_sets[i], L0, r0, N, cs, sn);
+                if (fp32) build_w32(SH, (int)i, L0, r0, N, cs, sn);
+                else SH.set_rho[i] = build_frags(SH, (int)i, L0, r0, N, cs, sn);
             } else if (i < nset + mid_rep.size()) {
                 const size_t k = i - nset, q = mid_rep[k];
                 build_mid(SH, sides[live[q]], spl[q].mid, SH.mid + mid_off[k]);
@@ -1216,11 +1221,9 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             }
         };
         parallel(nset + mid_rep.size() + (fp32 ? suf_rep.size() : 0), fill_one);
-        if (!fp32)
-            for (size_t i = 0; i < nset; ++i) SH.rho[SH.frag_sets[i]] = set_rho[i];
         parallel(live.size(), [&](size_t q) {
             add_side(T, slot[q], SH, sides[live[q]], spl[q], rmax, order, paf, group[live[q]], fp32,
-                     mid_off[side_mid[q]], suf_off[side_suf[q]]);
+                     mid_off[side_mid[q]], suf_off[side_suf[q]], side_set[q]);
         });
     }
     return prep;
