@@ -91,7 +91,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         // while the device runs batch i, one batch at a time (each preparation
         // has every host thread); the first batch is a quarter of the budget and
         // the second half of it, so that the device starts early and the host
-        // keeps ahead of it.
+        // keeps ahead of it, and the last at most a quarter.
         static_assert(std::is_trivially_copyable_v<SideHost> && std::is_trivially_destructible_v<SideHost>);
         if (ctx->unc_side_buf.size() < (size_t)(2 * P) * sizeof(SideHost))
             ctx->unc_side_buf.resize((size_t)(2 * P) * sizeof(SideHost));
@@ -166,6 +166,21 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 }
                 batches.emplace_back(q0, e);
                 q0 = e;
+            }
+            // a last batch of at most a quarter of the budget: its pairs are
+            // ordered after the device is done, the others' while it works
+            uint64_t last_leaves = 0;
+            if (!batches.empty())
+                for (int64_t q = batches.back().first; q < batches.back().second; ++q) last_leaves += pair_leaves(q);
+            if (batches.size() > 1 && last_leaves > leaf_budget / 4) {
+                const auto [l0, l1] = batches.back();
+                uint64_t tail = 0;
+                int64_t cut = l1;
+                while (cut > l0 + 1 && tail + pair_leaves(cut - 1) <= leaf_budget / 4) tail += pair_leaves(--cut);
+                if (cut > l0 && cut < l1) {
+                    batches.back().second = cut;
+                    batches.emplace_back(cut, l1);
+                }
             }
             if (batches.size() > 1) after = prepare_after(next, batches[1].first, batches[1].second);
         }
