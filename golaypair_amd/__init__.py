@@ -213,6 +213,25 @@ def cand_line(word, key):
     return "PSD" + digits + " " + "".join("%d " % v for v in word)
 
 
+BINARY_MAGIC = b"GPCB"
+
+
+def read_binary_cands(path):
+    """A candidate file written with the CLI's --binary option: "GPCB", version
+    1, n, K = n // 2, a zero byte, the record count (u64 little endian), then
+    per record n int8 letters and K uint8 key values (the numbers printed after
+    "PSD" in the text form).  Returns Cands; Cands.lines() gives the text form."""
+    data = open(path, "rb").read()
+    if len(data) < 16 or data[:4] != BINARY_MAGIC or data[4] != 1:
+        raise ValueError("%s: not a binary candidate file" % path)
+    n, K = data[5], data[6]
+    m = int.from_bytes(data[8:16], "little")
+    if K != n // 2 or len(data) != 16 + m * (n + K):
+        raise ValueError("%s: bad header or size" % path)
+    rec = np.frombuffer(data, np.uint8, offset=16).reshape(m, n + K)
+    return Cands(rec[:, :n].view(np.int8), rec[:, n:])
+
+
 def pair_line(a, b):
     """Pair line text without newline (io.cpp:75-95)."""
     return "".join("%d " % v for v in a) + " " + "".join("%d " % v for v in b)

@@ -215,6 +215,59 @@ bool parse_cands(const std::string& text, int n, int bound, bool prefer_inverted
     return true;
 }
 
+// Compact binary candidate files (--binary): "GPCB", format version 1, n,
+// K = n/2, a zero byte, the record count (8 bytes, little endian), then per
+// candidate its n letters (int8) and its K key values (uint8, the numbers the
+// text form prints after "PSD").  About a third of the text size at n = 15;
+// the text form stays the default (byte-identical to the reference's files).
+constexpr char kBinMagic[4] = {'G', 'P', 'C', 'B'};
+
+std::string cands_binary(const Cands& c) {
+    const int K = c.n / 2;
+    const uint64_t m = (uint64_t)c.size();
+    std::string s(16 + m * (c.n + K), '\0');
+    std::memcpy(&s[0], kBinMagic, 4);
+    s[4] = 1;
+    s[5] = (char)c.n;
+    s[6] = (char)K;
+    for (int b = 0; b < 8; ++b) s[8 + b] = (char)((m >> (8 * b)) & 0xFF);
+    char* p = &s[16];
+    for (uint64_t i = 0; i < m; ++i) {
+        std::memcpy(p, &c.words[i * c.n], c.n);
+        std::memcpy(p + c.n, &c.keys[i * K], K);
+        p += c.n + K;
+    }
+    return s;
+}
+
+bool is_binary_cands(const std::string& data) { return data.size() >= 16 && std::memcmp(data.data(), kBinMagic, 4) == 0; }
+
+bool parse_binary_cands(const std::string& data, int n, Cands& out, std::string& err) {
+    const int K = n / 2;
+    uint64_t m = 0;
+    for (int b = 0; b < 8; ++b) m |= (uint64_t)(uint8_t)data[8 + b] << (8 * b);
+    if (data[4] != 1 || (uint8_t)data[5] != n || (uint8_t)data[6] != K || data.size() != 16 + m * (n + K)) {
+        err = "binary candidate file: bad header or size";
+        return false;
+    }
+    out.n = n;
+    out.words.resize(m * n);
+    out.keys.resize(m * K);
+    const char* p = data.data() + 16;
+    for (uint64_t i = 0; i < m; ++i) {
+        std::memcpy(&out.words[i * n], p, n);
+        std::memcpy(&out.keys[i * K], p + n, K);
+        p += n + K;
+    }
+    return true;
+}
+
+// Either form of a candidate file.
+bool parse_any_cands(const std::string& data, int n, int bound, bool prefer_inverted, Cands& out, std::string& err) {
+    if (is_binary_cands(data)) return parse_binary_cands(data, n, out, err);
+    return parse_cands(data, n, bound, prefer_inverted, out, err);
+}
+
 // PairReader (io.cpp:39-73): one pair per line, exactly 2*len integers.
 bool read_pairs_lines(const std::string& path, int len, Pairs& out) {
     out.n = len;
@@ -261,7 +314,13 @@ int main(int argc, char** argv) {
     std::string sort_in = "filtered", sort_out = "filtered_sorted";
     std::string match_in = "filtered_sorted", match_out = "pairs";
     bool match_append = false;
+    bool binary = false;  // --binary: compact binary candidate files
     std::string unc_in = "pairs", unc_out = "pairs", unc_tmp = "";
+    // input lines to uncompress (main.cpp:110-113,192-196 declare these; the
+    // reference ignores them): --line (1-indexed) or the 0-indexed range
+    // [begin, end) with step, as script/uncompress.sh uses them to shard
+    long long unc_line = 0;
+    unsigned long long unc_begin = 0, unc_end = ~0ull, unc_step = 1;
     std::string ctx_name;
     std::vector<std::string> args(argv + 1, argv + argc);
     for (size_t i = 0; i < args.size(); ++i) {
@@ -285,6 +344,7 @@ int main(int argc, char** argv) {
             while (i + 1 < args.size() && is_int(args[i + 1])) o.compress.push_back(std::stoi(args[++i]));
             if (o.compress.empty()) { std::cerr << "--compress requires at least one value\n"; return 106; }
         } else if (a == "--paf") { need(v); o.paf = std::stoi(v); }
+        else if (a == "--binary") binary = true;
         else if (a == "-j" || a == "--jobid") { need(v); o.job_id = std::stoi(v); }
         else if (a == "-n" || a == "--numjob") { need(v); o.job_count = std::stoi(v); }
         else if ((a == "-d" || a == "--device") && ctx_name == "uncompress") {
@@ -300,13 +360,16 @@ int main(int argc, char** argv) {
         else if ((a == "-i" || a == "--input") && (ctx_name == "uncompress" || ctx_name == "filter")) need(unc_in);
         else if ((a == "-o" || a == "--output") && (ctx_name == "uncompress" || ctx_name == "filter")) need(unc_out);
         else if ((a == "-p" || a == "--prefix") && ctx_name == "uncompress") need(unc_tmp);
-        else if ((a == "--internal" || a == "--sorted" || a == "-l" || a == "--line" || a == "-b" ||
-                  a == "--range_begin" || a == "-e" || a == "--range_end" || a == "-s" || a == "--range_step") &&
-                 ctx_name == "uncompress") { need(v); }
+        else if ((a == "-l" || a == "--line") && ctx_name == "uncompress") { need(v); unc_line = std::stoll(v); }
+        else if ((a == "-b" || a == "--range_begin") && ctx_name == "uncompress") { need(v); unc_begin = std::stoull(v); }
+        else if ((a == "-e" || a == "--range_end") && ctx_name == "uncompress") { need(v); unc_end = std::stoull(v); }
+        else if ((a == "-s" || a == "--range_step") && ctx_name == "uncompress") { need(v); unc_step = std::stoull(v); }
+        else if ((a == "--internal" || a == "--sorted") && ctx_name == "uncompress") { need(v); }
         else if (is_int(a) && !o.have_order) { o.order = std::stoi(a); o.have_order = true; }
         else { std::cerr << "The following argument was not expected: " << a << "\n"; return 109; }
     }
     if (!o.have_order) { std::cerr << "order is required\n"; return 106; }
+    if (unc_step == 0 || unc_line < 0) { std::cerr << "--range_step must be >= 1 and --line >= 1\n"; return 105; }
     if (!(s_all || s_gen || s_sort || s_match || s_unc || s_filter)) { std::cerr << "A subcommand is required\n"; return 106; }
     g_stats = std::getenv("GOLAY_STATS") != nullptr;
     if (mkdirs(o.work_dir()) != 0) { std::cerr << "Failed to create temp directories\n"; return 1; }
@@ -346,14 +409,35 @@ int main(int argc, char** argv) {
             B.words.resize(cb.count * n); B.keys.resize(cb.count * (n / 2));
             break;
         }
-        write_file(cand(gen_prefix, 'a', o.job_id), cands_text(A));
-        write_file(cand(gen_prefix, 'b', o.job_id), cands_text(B));
+        write_file(cand(gen_prefix, 'a', o.job_id), binary ? cands_binary(A) : cands_text(A));
+        write_file(cand(gen_prefix, 'b', o.job_id), binary ? cands_binary(B) : cands_text(B));
         std::cout << "Passed PSD filter: " << (A.size() + B.size()) << "\n";
         report("gen", st);
     }
     if (s_sort || s_all) {  // main.cpp:69-79, sort.cpp:7-23
         std::cout << "Sorting Candidates\n";
         for (char ab : {'a', 'b'}) {
+            if (binary) {  // shards of either form -> one sorted, unique binary file
+                const int n = o.order / o.compress[0];
+                Cands all;
+                all.n = n;
+                for (int j = 0; j < o.job_count; ++j) {
+                    std::string t, err;
+                    Cands c;
+                    if (!read_file(cand(sort_in, ab, j), t) || !parse_any_cands(t, n, bound, ab == 'b', c, err)) {
+                        std::cerr << "Sort failed\n";
+                        return 1;
+                    }
+                    all.words.insert(all.words.end(), c.words.begin(), c.words.end());
+                    all.keys.insert(all.keys.end(), c.keys.begin(), c.keys.end());
+                }
+                gp_cands v = all.view();
+                if (all.size() > 0 && (rc = gp_sort_candidates(g_ctx, &v)) < 0) die_gp("gp_sort_candidates", rc);
+                all.words.resize(v.count * n);
+                all.keys.resize(v.count * (n / 2));
+                write_file(sorted_name(sort_out, ab), cands_binary(all));
+                continue;
+            }
             std::string all_text;
             for (int j = 0; j < o.job_count; ++j) {
                 std::string t;
@@ -371,7 +455,7 @@ int main(int argc, char** argv) {
         read_file(sorted_name(match_in, 'a'), ta);
         read_file(sorted_name(match_in, 'b'), tb);
         Cands A, B;
-        if (!parse_cands(ta, n, bound, false, A, err) || !parse_cands(tb, n, bound, true, B, err)) {
+        if (!parse_any_cands(ta, n, bound, false, A, err) || !parse_any_cands(tb, n, bound, true, B, err)) {
             std::cerr << "match: " << err << "\n";
             return 1;
         }
@@ -402,6 +486,20 @@ int main(int argc, char** argv) {
             const int N = o.order / o.compress[i + 1];
             Pairs in;
             read_pairs_lines(match_name(prefix, i), L, in);
+            if (i == 0 && (unc_line > 0 || unc_begin > 0 || unc_end != ~0ull || unc_step != 1)) {
+                // the selected lines of the first level's input, in file order
+                Pairs sel;
+                sel.n = in.n;
+                for (int64_t q = 0; q < in.size(); ++q) {
+                    const uint64_t u = (uint64_t)q;
+                    const bool take = unc_line > 0 ? q + 1 == unc_line
+                                                   : u >= unc_begin && u < unc_end && (u - unc_begin) % unc_step == 0;
+                    if (!take) continue;
+                    sel.a.insert(sel.a.end(), in.a.begin() + q * L, in.a.begin() + (q + 1) * L);
+                    sel.b.insert(sel.b.end(), in.b.begin() + q * L, in.b.begin() + (q + 1) * L);
+                }
+                in = std::move(sel);
+            }
             const std::string out_path = match_name(unc_out, i + 1);
             write_file(out_path, "");
             if (o.compress[i] <= 1) continue;
@@ -435,11 +533,24 @@ int main(int argc, char** argv) {
             write_file(out_path, pairs_text(P));
             if (in.size() > 0) {  // uncompression.cpp:45-48: the last pair's files stay behind
                 const std::string base = wd + "/" + unc_tmp + "uncompress_tmp_";
-                const std::string xa = cands_text(ta), xb = cands_text(tb);
-                write_file(base + "a", xa);
-                write_file(base + "b", xb);
-                write_file(base + "a.sorted", sorted_unique_text(xa));
-                write_file(base + "b.sorted", sorted_unique_text(xb));
+                if (binary) {
+                    write_file(base + "a", cands_binary(ta));
+                    write_file(base + "b", cands_binary(tb));
+                    for (Cands* c : {&ta, &tb}) {
+                        gp_cands v = c->view();
+                        if (c->size() > 0 && (rc = gp_sort_candidates(g_ctx, &v)) < 0) die_gp("gp_sort_candidates", rc);
+                        c->words.resize(v.count * N);
+                        c->keys.resize(v.count * (N / 2));
+                    }
+                    write_file(base + "a.sorted", cands_binary(ta));
+                    write_file(base + "b.sorted", cands_binary(tb));
+                } else {
+                    const std::string xa = cands_text(ta), xb = cands_text(tb);
+                    write_file(base + "a", xa);
+                    write_file(base + "b", xb);
+                    write_file(base + "a.sorted", sorted_unique_text(xa));
+                    write_file(base + "b.sorted", sorted_unique_text(xb));
+                }
             }
             report("uncompress", st);
         }

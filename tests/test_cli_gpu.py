@@ -45,6 +45,64 @@ def test_cli_all_matches_oracle(args):
         same_dirs(os.path.join(d1, "order-%d" % order), os.path.join(d2, "order-%d" % order))
 
 
+@pytest.mark.parametrize("flags,select", [
+    (["--range_begin", 1, "--range_step", 2], lambda i: i >= 1 and (i - 1) % 2 == 0),
+    (["-b", 0, "-e", 3], lambda i: i < 3),
+    (["--line", 2], lambda i: i == 1),
+])
+def test_cli_uncompress_slicing(flags, select):
+    """--line (1-indexed) and --range_begin/--range_end/--range_step (0-indexed
+    lines, main.cpp:110-113,192-196; script/uncompress.sh shards with them)
+    uncompress only the selected lines of the input pairs file: the output is
+    the oracle's for a file holding just those lines."""
+    order, comp = 26, [2, 1]
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        assert run(_oracle.CLI, [order, "--compress"] + comp + ["gen", "sort", "match"], d2).returncode == 0
+        wd1, wd2 = os.path.join(d1, "order-%d" % order), os.path.join(d2, "order-%d" % order)
+        name = "%d-pairs-%d" % (order, comp[0])
+        lines = open(os.path.join(wd2, name)).read().splitlines(True)
+        assert len(lines) > 4
+        os.makedirs(wd1)
+        open(os.path.join(wd1, name), "w").write("".join(lines))
+        r = run(g.CLI_PATH, [order, "--compress"] + comp + ["uncompress"] + flags, d1)
+        assert r.returncode == 0, r.stderr
+        open(os.path.join(wd2, name), "w").write("".join(l for i, l in enumerate(lines) if select(i)))
+        assert run(_oracle.CLI, [order, "--compress"] + comp + ["uncompress"], d2).returncode == 0
+        out = "%d-pairs-%d" % (order, comp[1])
+        assert open(os.path.join(wd1, out)).read() == open(os.path.join(wd2, out)).read()
+
+
+@pytest.mark.parametrize("args", [
+    [16, "--compress", 2, 1, "all"],
+    [21, "--compress", 3, 1, "--paf", -2, "all"],
+    [20, "--compress", 10, 5, 1, "all"],
+])
+def test_cli_binary_candidate_files(args):
+    """--binary writes the candidate files (gen shards, sorted files, the
+    uncompression temporary files) in the compact binary form; read back they
+    give the text files line for line, and the pair and unique files are the
+    same bytes as in text mode."""
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        assert run(g.CLI_PATH, args + ["--binary"], d1).returncode == 0
+        assert run(g.CLI_PATH, args, d2).returncode == 0
+        order = args[0]
+        w1, w2 = os.path.join(d1, "order-%d" % order), os.path.join(d2, "order-%d" % order)
+        assert sorted(os.listdir(w1)) == sorted(os.listdir(w2))
+        nbin = 0
+        for f in os.listdir(w1):
+            raw = open(os.path.join(w1, f), "rb").read()
+            text = open(os.path.join(w2, f)).read()
+            if raw[:4] == g.BINARY_MAGIC:
+                nbin += 1
+                got = "".join(l + "\n" for l in g.read_binary_cands(os.path.join(w1, f)).lines())
+                assert got == text, f
+                if text.count("\n") >= 16:  # the 16-byte header aside, records are smaller than lines
+                    assert len(raw) < len(text.encode()), f
+            else:
+                assert raw == text.encode(), f
+        assert nbin >= 4
+
+
 def test_cli_c1_golden():
     with tempfile.TemporaryDirectory() as d:
         assert run(g.CLI_PATH, [16, "--compress", 2, 1, "all"], d).returncode == 0
