@@ -8,6 +8,7 @@
 #include "gp_internal.hpp"
 #include "gp_sort.hpp"
 #include "psd_core.hpp"
+#include "candkey.hpp"
 #include "linesort.hpp"
 #include "textfmt.hpp"
 
@@ -99,22 +100,32 @@ int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
         if (m == 0) return GP_OK;
         const int n = s.n, K = s.nkey;
         if (K != n / 2) throw ArgError("gp_sort_candidates: nkey must be n/2");
-        std::string text;
-        std::vector<size_t> off;
-        ph.next("sort: lines");
-        build_lines(m, [&](std::string& t, int64_t i) { append_cand_line(t, &s.words[i * n], &s.keys[i * K], n); },
-                    text, off);
+        // fixed-width keys in line order (no text): sorted, then equal keys
+        // (equal lines) dropped
+        ph.next("sort: keys");
+        const CandKeyer ck(n);
+        const int W = ck.W;
+        std::vector<uint64_t> key((size_t)m * W);
+        parallel_chunks(m, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i) ck.key(&s.words[i * n], &s.keys[i * K], &key[(size_t)i * W]);
+        });
+        ph.next("sort: sort");
+        auto kcmp = [&](int64_t x, int64_t y) {
+            const uint64_t* a = &key[(size_t)x * W];
+            const uint64_t* b = &key[(size_t)y * W];
+            for (int q = 0; q < W; ++q)
+                if (a[q] != b[q]) return a[q] < b[q] ? -1 : 1;
+            return 0;
+        };
         std::vector<int64_t> idx(m);
         std::iota(idx.begin(), idx.end(), 0);
-        auto line = [&](int64_t i) { return std::string_view(text.data() + off[i], off[i + 1] - off[i]); };
-        ph.next("sort: sort");
-        sort_lines(LineSet{&text, &off, 3}, idx);  // every line starts with "PSD"
+        parallel_sort(idx, [&](int64_t x, int64_t y) { return kcmp(x, y) < 0; });
         ph.next("sort: uniq + copy");
         // uniq: a line equal to its predecessor is dropped; kept rows are
         // numbered by a prefix sum over chunks and copied on host threads
         std::vector<uint8_t> keep(m);
         parallel_chunks(m, [&](int64_t r0, int64_t r1) {
-            for (int64_t r = r0; r < r1; ++r) keep[r] = r == 0 || line(idx[r]) != line(idx[r - 1]);
+            for (int64_t r = r0; r < r1; ++r) keep[r] = r == 0 || kcmp(idx[r], idx[r - 1]) != 0;
         });
         std::vector<int64_t> dst(m);
         int64_t w = 0;

@@ -11,6 +11,7 @@
 #include "gp_host.hpp"
 #include "gp_internal.hpp"
 #include "gp_unc.hpp"
+#include "candkey.hpp"
 #include "linesort.hpp"
 #include "textfmt.hpp"
 #include "threadpool.hpp"
@@ -191,18 +192,17 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         uint64_t ord_ties = 0, ord_pairs = 0;
         auto order_batch = [&, N](UncEmitted& E) {
             const int K = N / 2;
-            Phase ph("order: lines");
-            std::string ta, tb;
-            std::vector<size_t> oa, ob;
-            build_lines(E.count, [&](std::string& t, int64_t i) {
-                append_cand_line(t, &E.words[i * N], &E.keys[i * K], N);
-            }, ta, oa);
-            build_lines(E.count, [&](std::string& t, int64_t i) {
-                append_cand_line(t, &E.words[(E.count + i) * N], &E.keys[(E.count + i) * K], N);
-            }, tb, ob);
-            const LineSet LA{&ta, &oa, 3}, LB{&tb, &ob, 3};
-            std::vector<uint64_t> ha, la;
-            line_keys(LA, E.count, ha, la);
+            Phase ph("order: keys");
+            // fixed-width keys in line order (candkey.hpp) of the A and B rows
+            const CandKeyer ck(N);
+            const int W = ck.W;
+            std::vector<uint64_t> ka((size_t)E.count * W), kb((size_t)E.count * W);
+            parallel_chunks(E.count, [&](int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; ++i) {
+                    ck.key(&E.words[i * N], &E.keys[i * K], &ka[(size_t)i * W]);
+                    ck.key(&E.words[(E.count + i) * N], &E.keys[(E.count + i) * K], &kb[(size_t)i * W]);
+                }
+            }, 256);
             ph.next("order: sort");
             std::vector<int64_t> ord;
             for (int64_t i = 0; i < E.count; ++i) {
@@ -213,12 +213,18 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 }
                 ord.push_back(i);
             }
+            auto kless = [W](const uint64_t* x, const uint64_t* y) {
+                for (int q = 0; q < W; ++q)
+                    if (x[q] != y[q]) return x[q] < y[q];
+                return false;
+            };
             std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
                 if (E.group[x] != E.group[y]) return E.group[x] < E.group[y];
-                if (ha[x] != ha[y]) return ha[x] < ha[y];
-                if (la[x] != la[y]) return la[x] < la[y];
-                const std::string_view ax = LA.line(x), ay = LA.line(y);
-                return ax != ay ? ax < ay : LB.line(x) < LB.line(y);
+                const uint64_t* ax = &ka[(size_t)x * W];
+                const uint64_t* ay = &ka[(size_t)y * W];
+                if (kless(ax, ay)) return true;
+                if (kless(ay, ax)) return false;
+                return kless(&kb[(size_t)x * W], &kb[(size_t)y * W]);
             });
             ph.next("order: append");
             for (int64_t i : ord) {

@@ -2,7 +2,7 @@
 // under LC_ALL=C (src/core/sort/sort.cpp:17): each line gets a 16-byte
 // big-endian prefix key (zero padded, so a proper prefix sorts first, every
 // text byte being >= 0x20), and ties on the key fall back to comparing the rest
-// of the lines.  Chunks are sorted on host threads and merged pairwise.
+// of the lines.  Sorted on host threads by a sample sort.
 #pragma once
 
 #include <stdint.h>
@@ -60,22 +60,56 @@ void parallel_chunks(int64_t m, F f, int64_t grain = 4096, int threads = 16) {
     pool().parallel_for(nt, [&](size_t t) { f(m * (int64_t)t / nt, m * (int64_t)(t + 1) / nt); });
 }
 
-// Sorts idx with `less` on host threads: chunks sorted concurrently, then merged pairwise.
+// Sorts v with `less` on host threads (sample sort): p - 1 splitters from a
+// regular sample cut v into p buckets (counted and scattered per chunk), and
+// the buckets are sorted concurrently, so no pass is serial over all of v.
+// Not stable: elements that compare equal may come out in any order.
 template <class T, class Less>
-void parallel_sort(std::vector<T>& idx, const Less& less, int threads = 16) {
-    const int64_t m = (int64_t)idx.size();
-    const int nt = std::max(1, std::min<int>(threads, (int)(m / 4096)));
-    std::vector<int64_t> cut(nt + 1);
-    for (int t = 0; t <= nt; ++t) cut[t] = m * t / nt;
-    pool().parallel_for(nt, [&](size_t t) { std::sort(idx.begin() + cut[t], idx.begin() + cut[t + 1], less); });
-    for (int w = 1; w < nt; w *= 2) {
-        const int pairs = (nt - w + 2 * w - 1) / (2 * w);
-        pool().parallel_for(pairs, [&](size_t q) {
-            const int t = (int)q * 2 * w;
-            const int64_t a = cut[t], b = cut[t + w], e = cut[std::min(t + 2 * w, nt)];
-            std::inplace_merge(idx.begin() + a, idx.begin() + b, idx.begin() + e, less);
-        });
+void parallel_sort(std::vector<T>& v, const Less& less, int threads = 16) {
+    const int64_t m = (int64_t)v.size();
+    const int p = (int)std::max<int64_t>(1, std::min<int64_t>(threads, m / 4096));
+    if (p == 1) {
+        std::sort(v.begin(), v.end(), less);
+        return;
     }
+    // splitters: every 32nd element of a sorted sample of 32 p evenly spaced elements
+    const int per = 32;
+    std::vector<T> sample;
+    sample.reserve((size_t)per * p);
+    for (int s = 0; s < per * p; ++s) sample.push_back(v[(size_t)(m * (int64_t)s / (per * p))]);
+    std::sort(sample.begin(), sample.end(), less);
+    std::vector<T> split;
+    for (int b = 1; b < p; ++b) split.push_back(sample[(size_t)b * per]);
+    auto bucket_of = [&](const T& x) {
+        return (int)(std::upper_bound(split.begin(), split.end(), x, less) - split.begin());
+    };
+    // per chunk and bucket counts, then the scatter offsets
+    const int nc = p;
+    std::vector<int64_t> cut(nc + 1);
+    for (int c = 0; c <= nc; ++c) cut[c] = m * c / nc;
+    std::vector<int> bk((size_t)m);
+    std::vector<int64_t> cnt((size_t)nc * p, 0);
+    pool().parallel_for(nc, [&](size_t c) {
+        for (int64_t i = cut[c]; i < cut[c + 1]; ++i) {
+            bk[i] = bucket_of(v[i]);
+            ++cnt[c * p + bk[i]];
+        }
+    });
+    std::vector<int64_t> start((size_t)p + 1, 0), pos((size_t)nc * p);
+    for (int b = 0; b < p; ++b) {
+        int64_t acc = start[b];
+        for (int c = 0; c < nc; ++c) {
+            pos[(size_t)c * p + b] = acc;
+            acc += cnt[(size_t)c * p + b];
+        }
+        start[b + 1] = acc;
+    }
+    std::vector<T> out((size_t)m);
+    pool().parallel_for(nc, [&](size_t c) {
+        for (int64_t i = cut[c]; i < cut[c + 1]; ++i) out[pos[c * p + bk[i]]++] = v[i];
+    });
+    pool().parallel_for(p, [&](size_t b) { std::sort(out.begin() + start[b], out.begin() + start[b + 1], less); });
+    v.swap(out);
 }
 
 inline void line_keys(const LineSet& L, int64_t m, std::vector<uint64_t>& hi, std::vector<uint64_t>& lo) {
