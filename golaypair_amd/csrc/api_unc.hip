@@ -90,9 +90,9 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         // tables are prepared while the other sides are built.  Batches by leaf
         // budget; the host tables of batch i + 2 are prepared on another thread
         // while the device runs batch i, one batch at a time (each preparation
-        // has every host thread); the first batch is a quarter of the budget and
-        // the second half of it, so that the device starts early and the host
-        // keeps ahead of it, and the last at most a quarter.
+        // has every host thread); the first three batches are an eighth, a
+        // quarter and half of the budget, so that the device starts early and
+        // the host keeps ahead of it, and the last is at most a quarter.
         static_assert(std::is_trivially_copyable_v<SideHost> && std::is_trivially_destructible_v<SideHost>);
         if (ctx->unc_side_buf.size() < (size_t)(2 * P) * sizeof(SideHost))
             ctx->unc_side_buf.resize((size_t)(2 * P) * sizeof(SideHost));
@@ -141,7 +141,7 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 built = c1;
                 for (; q1 < built; ++q1) {
                     const uint64_t add = pair_leaves(q1);
-                    if (q1 > 0 && leaves + add > leaf_budget / 4) {
+                    if (q1 > 0 && leaves + add > leaf_budget / 8) {
                         closed = true;
                         break;
                     }
@@ -159,8 +159,11 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
                 int64_t e = q0;
                 while (e < P) {
                     const uint64_t add = pair_leaves(e);
-                    // the second batch is half the budget: its tables are ready sooner
-                    const uint64_t cap = batches.size() == 1 ? leaf_budget / 2 : leaf_budget;
+                    // the second and third batches are a quarter and half the budget:
+                    // their tables are ready sooner
+                    const uint64_t cap = batches.size() == 1   ? leaf_budget / 4
+                                         : batches.size() == 2 ? leaf_budget / 2
+                                                               : leaf_budget;
                     if (e > q0 && lv + add > cap) break;
                     lv += add;
                     ++e;
