@@ -2,8 +2,8 @@
 
 Decisions must be bit-exact (the kernels settle every candidate inside their
 stated error band in FP64); the kernels' own max-PSD values must lie within the
-error bound stated in DESIGN.md (|PSD' - PSD| <= eps(n, c), checked here with
-the bound recomputed from the table constants).
+error bound of the split-BF16 and FP32 paths (|PSD' - PSD| <= eps(n, c), with
+the bound recomputed here from the table constants).
 """
 import numpy as np
 import pytest
