@@ -329,15 +329,19 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     std::vector<GenSeg> segs;
     std::vector<GenTask> tasks;
     int64_t total = 0;
-    const int per_lane = 128;
     for (auto& [J, side] : side_of_J) {
         GenSeg s{J, side, (int64_t)cnt[n * cstride + J]};
         if ((uint64_t)s.count >= cap) throw ArgError("gen: search space too large");
-        const int id = (int)segs.size();
         segs.push_back(s);
-        for (int64_t r = 0; r < s.count; r += 64 * per_lane) tasks.push_back(GenTask{id, per_lane, r, s.count});
         total += s.count;
     }
+    // ranks per lane: about 8 wave tasks per resident wave (so that the last
+    // round of tasks is short), at least 16 ranks to amortise each lane's unrank
+    const int64_t target_tasks = (int64_t)ctx->num_cus * 32 * 8;
+    const int per_lane = (int)std::clamp<int64_t>((total + 64 * target_tasks - 1) / (64 * target_tasks), 16, 128);
+    for (size_t id = 0; id < segs.size(); ++id)
+        for (int64_t r = 0; r < segs[id].count; r += 64 * per_lane)
+            tasks.push_back(GenTask{(int32_t)id, per_lane, r, segs[id].count});
     GenParams P{};
     P.n = n;
     P.k = k;
