@@ -90,21 +90,82 @@ __device__ __forceinline__ void wave_min(uint64_t (&k)[W]) {
     }
 }
 
-// One pair per wave.  Phase 1: the lanes share the a-part transforms
-// (swap, unit, kappa, sign, shift) and reduce the minimum a-part key.  Phase 2:
-// every transform reaching that minimum (found by a ballot over the same
-// enumeration) is broadcast, and the lanes share its b-part transforms
-// (e, sign, shift) for the minimum b-part key.  canon[q] = (a key, b key).
+// Least rotation (the start r of the lexicographically smallest rotation) of
+// the lane's sequence z[0..n) held in LDS at z[j * 64] (signed order); the
+// two-pointer minimum-expression method, O(n).
+__device__ __forceinline__ int least_rotation(const int8_t* z, int n) {
+    int i = 0, j = 1, k = 0;
+    while (i < n && j < n && k < n) {
+        int x = i + k, y = j + k;
+        if (x >= n) x -= n;
+        if (y >= n) y -= n;
+        const int a = z[x * 64], b = z[y * 64];
+        if (a == b) {
+            ++k;
+            continue;
+        }
+        if (a > b) i += k + 1;
+        else j += k + 1;
+        if (i == j) ++j;
+        k = 0;
+    }
+    return i < j ? i : j;
+}
+
+// Writes z[j] = sigma * (-1)^(kappa j) * X[(u j) mod n] (lane's column of the
+// LDS sequence area); kappa is used only for even n (altnegative).
+__device__ __forceinline__ void decimated(const int8_t* X, int n, int u, int kappa, int sigma, int8_t* z) {
+    int idx = 0;
+    for (int j = 0; j < n; ++j) {
+        int v = X[idx] * sigma;
+        if (kappa & j & 1) v = -v;
+        z[j * 64] = (int8_t)v;
+        idx += u;
+        if (idx >= n) idx -= n;
+    }
+}
+
+// Packed key of the rotation of z starting at r (layout of img_key).
+template <int W>
+__device__ __forceinline__ void rot_key(const int8_t* z, const FilterParams& P, int r, uint64_t (&k)[W]) {
+    const int n = P.n;
+    int idx = r, i = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t acc = 0;
+        const int iend = min(n, (w + 1) * P.E);
+        const int cnt = max(0, iend - i);
+        for (; i < iend; ++i) {
+            acc = (acc << P.B) | (uint64_t)((z[idx * 64] + P.mx) >> P.sh);
+            if (++idx == n) idx = 0;
+        }
+        k[w] = cnt ? acc << (P.B * (P.E - cnt)) : 0ull;
+    }
+}
+
+// One pair per wave; the orbit minimum by construction.  Every transform acts
+// as image[i] = s (-1)^(kappa i) X[(u i + t) mod n] on each half.  With
+// t = u s' this is a rotation by s' of z[j] = (-1)^(kappa j) X[(u j) mod n]
+// (for kappa = 1, n is even and (-1)^(kappa i) = (-1)^(s') (-1)^(kappa (i + s')),
+// so the rotation's sign is s (-1)^(s'), both signs reachable since s is free
+// whenever kappa is).  So for each (swap, u, kappa, sign) the minimum over t
+// is the least rotation of +-z.  Phase 1: lanes over the a-part variants
+// (swap, u, kappa, sign); the wave minimum is the a key.  Phase 2: every lane
+// whose variant reaches it takes the least rotations of its b-part variants
+// (unit e u with e = +-1, the same kappa, sign free with altnegative), which
+// do not depend on the a-part's shift or sign; the wave minimum is the b key.
+// canon[q] = (a key, b key), as the minimum over the whole group.
 template <int W>
 __global__ void __launch_bounds__(256) k_orbit_min(FilterParams P, const int8_t* __restrict__ pa,
                                                    const int8_t* __restrict__ pb, int64_t m,
                                                    uint64_t* __restrict__ canon) {
     __shared__ int8_t sx[4][2][kMaxN];
+    __shared__ int8_t zs[4][kMaxN * 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int n = P.n;
     const int nk = P.alt ? 2 : 1, ns = P.alt ? 2 : 1;
-    const int ncand = 2 * P.nunits * nk * ns * n;
-    const int nbimg = 2 * ns * n;
+    const int nvar = 2 * P.nunits * nk * ns;
+    int8_t* z = &zs[wv][lane];
     for (int64_t q = (int64_t)blockIdx.x * 4 + wv; q < m; q += (int64_t)gridDim.x * 4) {
         if (lane < n) {
             sx[wv][0][lane] = pa[q * n + lane];
@@ -112,66 +173,49 @@ __global__ void __launch_bounds__(256) k_orbit_min(FilterParams P, const int8_t*
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        auto decode = [&](int c, int& sw, int& u, int& kappa, int& sg, int& t) {
-            t = c % n;
-            c /= n;
+        // variant c -> (swap, unit index, kappa, sign)
+        auto decode = [&](int c, int& sw, int& ui, int& kappa, int& sg) {
             sg = (c % ns) ? -1 : 1;
             c /= ns;
             kappa = c % nk;
             c /= nk;
-            u = P.units[c % P.nunits];
+            ui = c % P.nunits;
             sw = c / P.nunits;
         };
-        uint64_t ka[W];
+        uint64_t ka[W], kb[W], lmin[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) ka[w] = ~0ull;
-        for (int c = lane; c < ncand; c += 64) {
-            int sw, u, kappa, sg, t;
-            decode(c, sw, u, kappa, sg, t);
+        for (int w = 0; w < W; ++w) ka[w] = kb[w] = ~0ull;
+        for (int c = lane; c < nvar; c += 64) {
+            int sw, ui, kappa, sg;
+            decode(c, sw, ui, kappa, sg);
+            decimated(sx[wv][sw], n, P.units[ui], kappa, sg, z);
             uint64_t k[W];
-            img_key<W>(sx[wv][sw], P, u, t, kappa, sg, k);
+            rot_key<W>(z, P, least_rotation(z, n), k);
             if (key_less(k, ka))
 #pragma unroll
                 for (int w = 0; w < W; ++w) ka[w] = k[w];
         }
-        uint64_t lmin[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) lmin[w] = ka[w];
         wave_min<W>(ka);
-        // only a lane whose own minimum is the wave minimum holds transforms reaching it
         const bool mine = key_eq(lmin, ka);
-        uint64_t kb[W];
+        // phase 2: the lanes holding a minimising variant (re-found by recomputing)
+        for (int c = lane; c < nvar && mine; c += 64) {
+            int sw, ui, kappa, sg;
+            decode(c, sw, ui, kappa, sg);
+            decimated(sx[wv][sw], n, P.units[ui], kappa, sg, z);
+            uint64_t k[W];
+            rot_key<W>(z, P, least_rotation(z, n), k);
+            if (!key_eq(k, ka)) continue;
+            const int u = P.units[ui];
+            for (int e = 0; e < 2; ++e) {
+                const int ub = e ? (n - u) % n : u;
+                for (int sb = 0; sb < ns; ++sb) {
+                    decimated(sx[wv][sw ? 0 : 1], n, n == 1 ? 0 : ub, kappa, sb ? -1 : 1, z);
+                    rot_key<W>(z, P, least_rotation(z, n), k);
+                    if (key_less(k, kb))
 #pragma unroll
-        for (int w = 0; w < W; ++w) kb[w] = ~0ull;
-        {
-            for (int c0 = 0; c0 < ncand; c0 += 64) {
-                const int c = c0 + lane;
-                bool hit = false;
-                if (mine && c < ncand) {
-                    int sw, u, kappa, sg, t;
-                    decode(c, sw, u, kappa, sg, t);
-                    uint64_t k[W];
-                    img_key<W>(sx[wv][sw], P, u, t, kappa, sg, k);
-                    hit = key_eq(k, ka);
-                }
-                unsigned long long hits = __ballot(hit);
-                while (hits) {
-                    const int bit = __builtin_ctzll(hits);
-                    hits &= hits - 1;
-                    int sw, u, kappa, sg, t;
-                    decode(c0 + bit, sw, u, kappa, sg, t);
-                    const int8_t* Y = sx[wv][sw ? 0 : 1];
-                    for (int j = lane; j < nbimg; j += 64) {
-                        const int tb = j % n;
-                        const int sgb = ((j / n) % ns) ? -1 : 1;
-                        const int e = j / (n * ns);
-                        const int ub = e ? (n - u) % n : u;
-                        uint64_t k[W];
-                        img_key<W>(Y, P, n == 1 ? 0 : ub, tb, kappa, sgb, k);
-                        if (key_less(k, kb))
-#pragma unroll
-                            for (int w = 0; w < W; ++w) kb[w] = k[w];
-                    }
+                        for (int w = 0; w < W; ++w) kb[w] = k[w];
                 }
             }
         }
