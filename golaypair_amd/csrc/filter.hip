@@ -249,11 +249,27 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     if (n == 1) { P.nunits = 1; P.units[0] = 0; }
     // key packing: the values and their negations lie in [-mx, mx]; halve when all have mx's parity
     int mx = 0, odd_or = 0, odd_and = 1;  // max |v|; whether some / every value is odd
-    for (int64_t i = 0; i < m * n; ++i) {
-        const int a = in.a[i], b = in.b[i];
-        mx = std::max(mx, std::max(a < 0 ? -a : a, b < 0 ? -b : b));
-        odd_or |= (a | b) & 1;
-        odd_and &= a & b & 1;
+    {
+        const int nc = 16;
+        std::vector<int> cmx(nc, 0), cor(nc, 0), cand(nc, 1);
+        const int64_t tot = m * n;
+        pool().parallel_for(nc, [&](size_t c) {
+            int lmx = 0, lor = 0, land = 1;
+            for (int64_t i = tot * (int64_t)c / nc; i < tot * (int64_t)(c + 1) / nc; ++i) {
+                const int a = in.a[i], b = in.b[i];
+                lmx = std::max(lmx, std::max(a < 0 ? -a : a, b < 0 ? -b : b));
+                lor |= (a | b) & 1;
+                land &= a & b & 1;
+            }
+            cmx[c] = lmx;
+            cor[c] = lor;
+            cand[c] = land;
+        });
+        for (int c = 0; c < nc; ++c) {
+            mx = std::max(mx, cmx[c]);
+            odd_or |= cor[c];
+            odd_and &= cand[c];
+        }
     }
     // every v + mx even: all values odd with mx odd, or all even with mx even
     const bool same_parity = (mx & 1) ? odd_and == 1 : odd_or == 0;
@@ -292,17 +308,23 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
     // one row per pair: the canonical key (2W words) then the pair key (PW
     // words), so that (class, pair order) is one lexicographic word compare
     const int PW = (2 * n + 7) / 8, CW = 2 * W, RW = CW + PW;
-    std::vector<uint64_t> rk((size_t)m * RW, 0);
+    std::vector<uint64_t> rk((size_t)m * RW);
     parallel_chunks(m, [&](int64_t i0, int64_t i1) {
         for (int64_t i = i0; i < i1; ++i) {
             uint64_t* r = &rk[i * RW];
             for (int w = 0; w < CW; ++w) r[w] = canon[i * CW + w];
-            for (int j = 0; j < 2 * n; ++j) {
-                const int v = j < n ? in.a[i * n + j] : in.b[i * n + j - n];
-                r[CW + j / 8] |= (uint64_t)(uint8_t)(v + 128) << (56 - 8 * (j % 8));
+            const int8_t* pa = &in.a[i * n];
+            const int8_t* pb = &in.b[i * n];
+            for (int w = 0; w < PW; ++w) {
+                uint64_t acc = 0;
+                for (int j = 8 * w; j < 8 * w + 8; ++j) {
+                    const int v = j < n ? pa[j] : j < 2 * n ? pb[j - n] : -128;  // -128 packs as 0
+                    acc = (acc << 8) | (uint64_t)(uint8_t)(v + 128);
+                }
+                r[CW + w] = acc;
             }
         }
-    });
+    }, 1024);
     auto words_less = [&](int64_t x, int64_t y, int w0, int w1) {
         for (int w = w0; w < w1; ++w)
             if (rk[x * RW + w] != rk[y * RW + w]) return rk[x * RW + w] < rk[y * RW + w];

@@ -22,6 +22,7 @@
 
 #include "gp_internal.hpp"
 #include "gp_host.hpp"
+#include "gp_sort.hpp"
 #include "linesort.hpp"
 #include "psd_core.hpp"
 
@@ -276,6 +277,30 @@ __global__ void __launch_bounds__(256) k_gen_emit(const int8_t* __restrict__ wor
     if (t) atomicAdd(ties, t);
 }
 
+// Sort keys of the accepted rows in the reference's file order, A rows first:
+// side << 63 | the letter indices (v - a0) / 2, 4 bits each, first letter
+// most significant (n <= 15).
+__global__ void k_gen_sortkey(const int8_t* __restrict__ ow, const uint8_t* __restrict__ oside, int64_t m, int n,
+                              int a0, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = 0;
+        for (int j = 0; j < n; ++j) k = (k << 4) | (uint64_t)((ow[i * n + j] - a0) >> 1);
+        keys[i] = ((uint64_t)(oside[i] != 0) << 63) | k;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+// Rows permuted into sorted order.
+__global__ void k_gen_gather(const int8_t* __restrict__ ow, const uint8_t* __restrict__ okeys,
+                             const uint32_t* __restrict__ order, int64_t m, int n, int K, int8_t* __restrict__ w2,
+                             uint8_t* __restrict__ k2) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = order[i];
+        for (int j = 0; j < n; ++j) w2[i * n + j] = ow[s * n + j];
+        for (int k = 0; k < K; ++k) k2[i * K + k] = okeys[s * K + k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host
 
@@ -417,6 +442,7 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     const int stride = 16 * d.nb;
     int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)std::max<int64_t>(m, 1) * stride));
     uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)std::max<int64_t>(m, 1)));
+    bool sorted_on_device = false;  // rows already in file order, A rows first
     std::vector<int8_t> words_h;   // accepted rows only: [acc_n][n]
     std::vector<uint8_t> keys_h, side_h;
     int64_t acc_n = 0;
@@ -452,9 +478,30 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         words_h.resize((size_t)acc_n * n);
         keys_h.resize((size_t)acc_n * K);
         side_h.resize((size_t)acc_n);
+        const int8_t* src_w = d_ow;
+        const uint8_t* src_k = d_ok;
+        if (acc_n > 1 && n <= 15) {
+            // file order on the device: radix sort of (side, word) keys, rows gathered
+            uint64_t* d_sk = ctx->dev<uint64_t>("g_sk", (size_t)acc_n);
+            uint32_t* d_sv = ctx->dev<uint32_t>("g_sv", (size_t)acc_n);
+            uint64_t* d_tk = ctx->dev<uint64_t>("g_tk", (size_t)acc_n);
+            uint32_t* d_tv = ctx->dev<uint32_t>("g_tv", (size_t)acc_n);
+            uint32_t* d_hist = static_cast<uint32_t*>(ctx->buf("g_hist").ensure(radix_hist_bytes(acc_n)));
+            const unsigned g = (unsigned)std::min<int64_t>((acc_n + 255) / 256, (int64_t)ctx->num_cus * 8);
+            hipLaunchKernelGGL(k_gen_sortkey, dim3(g), dim3(256), 0, st, d_ow, d_os, acc_n, n, a0, d_sk, d_sv);
+            GP_HIP(hipGetLastError());
+            radix_sort_pairs(st, d_sk, d_sv, d_tk, d_tv, d_hist, acc_n, 0, 64);
+            int8_t* d_w2 = ctx->dev<int8_t>("g_ow2", (size_t)acc_n * n);
+            uint8_t* d_k2 = ctx->dev<uint8_t>("g_ok2", (size_t)acc_n * std::max(K, 1));
+            hipLaunchKernelGGL(k_gen_gather, dim3(g), dim3(256), 0, st, d_ow, d_ok, d_sv, acc_n, n, K, d_w2, d_k2);
+            GP_HIP(hipGetLastError());
+            src_w = d_w2;
+            src_k = d_k2;
+            sorted_on_device = true;
+        }
         if (acc_n > 0) {
-            GP_HIP(hipMemcpyAsync(words_h.data(), d_ow, words_h.size(), hipMemcpyDeviceToHost, st));
-            GP_HIP(hipMemcpyAsync(keys_h.data(), d_ok, keys_h.size(), hipMemcpyDeviceToHost, st));
+            GP_HIP(hipMemcpyAsync(words_h.data(), src_w, words_h.size(), hipMemcpyDeviceToHost, st));
+            GP_HIP(hipMemcpyAsync(keys_h.data(), src_k, keys_h.size(), hipMemcpyDeviceToHost, st));
             GP_HIP(hipMemcpyAsync(side_h.data(), d_os, side_h.size(), hipMemcpyDeviceToHost, st));
             GP_HIP(hipStreamSynchronize(st));
         }
@@ -465,6 +512,11 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     const int K = n / 2;
     std::vector<int64_t> ia, ib;
     for (int64_t i = 0; i < acc_n; ++i) (side_h[i] ? ib : ia).push_back(i);
+    if (sorted_on_device) {  // side_h is in emission order: only the A/B split counts are used
+        const int64_t na = (int64_t)ia.size();
+        std::iota(ia.begin(), ia.end(), (int64_t)0);
+        std::iota(ib.begin(), ib.end(), na);
+    }
     if (stats) stats->psd_pass += ia.size() + ib.size();
     // packed big-endian keys (byte v + 128 per letter): unsigned order = signed
     // lexicographic order; (key, row) records sorted directly, A and B concurrently
@@ -497,9 +549,11 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         else if (W == 2) run(std::integral_constant<int, 2>{});
         else run(std::integral_constant<int, 4>{});
     };
-    std::thread ta([&] { sort_side(ia); });
-    sort_side(ib);
-    ta.join();
+    if (!sorted_on_device) {
+        std::thread ta([&] { sort_side(ia); });
+        sort_side(ib);
+        ta.join();
+    }
     auto emit = [&](const std::vector<int64_t>& idx, CandSet& cs) {
         cs.n = n;
         cs.nkey = K;
