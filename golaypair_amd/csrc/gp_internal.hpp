@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -131,11 +132,13 @@ struct PinnedBuf {
 struct PinnedPool {
     std::mutex mu;
     std::vector<std::unique_ptr<PinnedBuf>> bufs;
+    size_t high = 0;  // largest request so far: a buffer that must grow grows to it
     PinnedBuf* acquire(size_t n) {
         std::lock_guard<std::mutex> lk(mu);
-        PinnedBuf* pick = nullptr;
+        high = std::max(high, n);
+        PinnedBuf* pick = nullptr;  // the smallest free buffer that fits
         for (auto& b : bufs)
-            if (!b->busy && b->bytes >= n) { pick = b.get(); break; }
+            if (!b->busy && b->bytes >= n && (!pick || b->bytes < pick->bytes)) pick = b.get();
         if (!pick)
             for (auto& b : bufs)
                 if (!b->busy) { pick = b.get(); break; }
@@ -147,7 +150,9 @@ struct PinnedPool {
             if (pick->p) GP_HIP(hipHostFree(pick->p));
             pick->p = nullptr;
             pick->bytes = 0;
-            const size_t want = n + n / 4 + 4096;
+            // sized for the largest request seen, so that requests of varying
+            // sizes stop reallocating once every buffer has grown
+            const size_t want = high + high / 4 + 4096;
             GP_HIP(hipHostMalloc(&pick->p, want, hipHostMallocDefault));
             pick->bytes = want;
         }
