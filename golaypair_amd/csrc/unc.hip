@@ -148,7 +148,7 @@ struct JoinCtl2 {
     unsigned long long checks;  // full PAF checks
     unsigned long long nb;      // B rows (counted by k_unc_rows when `jctl` is given)
     unsigned long long slots;   // table slots read by the probes
-    unsigned long long ncand;   // candidate pairs (equal hashes, same group)
+    unsigned long long ncand;   // candidate pairs (equal hashes)
 };
 
 // Hash table size in use: the host's `tmask` + 1, or, with `jctl`, the
@@ -197,7 +197,8 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
             }
             h = mix_in(h, S.role ? (int64_t)paf_target - v : v);
         }
-        hashes[i] = splitmix64(h);
+        // the row's role in bit 0, so that insert and probe read one array
+        hashes[i] = (splitmix64(h) << 1) | (uint64_t)(S.role == 1);
     }
     if (jctl) wave_atomic_add(&jctl->nb, nb);
 }
@@ -211,35 +212,34 @@ __global__ void k_ht_fill(unsigned long long* __restrict__ table, uint64_t tmask
         table[i] = ~0ull;
 }
 
-__global__ void k_ht_insert(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
-                            const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
+// B rows (bit 0 of the hash set) go into the table, tagged with the upper hash bits.
+__global__ void k_ht_insert(const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
                             uint64_t tmask, const unsigned long long* dcount, int64_t dcap, const JoinCtl2* jctl) {
     m = list_count(dcount, m, dcap);
     tmask = table_mask(jctl, tmask);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        if (sides[surv[i] >> 40].role != 1) continue;
-        const uint64_t h = hashes[i];
+        const uint64_t hr = hashes[i];
+        if (!(hr & 1)) continue;  // B rows only
+        const uint64_t h = hr >> 1;
         const unsigned long long e = ((h & 0xFFFFFFFFull) << 32) | (uint64_t)(uint32_t)i;
         uint64_t slot = (h >> 24) & tmask;
         while (atomicCAS(&table[slot], kEmpty, e) != kEmpty) slot = (slot + 1) & tmask;
     }
 }
 
-// A rows probe the table of B rows: every B row of the same group with the
-// same PAF hash is a candidate pair (A row i, B row j), checked exactly by
-// k_pair_check.
-__global__ void k_ht_probe(const UncSide* __restrict__ sides, const uint64_t* __restrict__ surv,
-                           const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
+// A rows probe the table of B rows: every B row with the same PAF hash is a
+// candidate pair (A row i, B row j), checked (group and PAF, exactly) by
+// k_pair_check.  Hashes carry the row's role in bit 0.
+__global__ void k_ht_probe(const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
                            uint64_t tmask, uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int64_t ccap,
                            JoinCtl2* ctl, const unsigned long long* dcount, int64_t dcap, bool device_mask) {
     m = list_count(dcount, m, dcap);
     tmask = table_mask(device_mask ? ctl : nullptr, tmask);
     unsigned long long nslots = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t ra = surv[i];
-        const UncSide& SA = sides[ra >> 40];
-        if (SA.role != 0) continue;
-        const uint64_t h = hashes[i];
+        const uint64_t hr = hashes[i];
+        if (hr & 1) continue;  // A rows only
+        const uint64_t h = hr >> 1;
         uint64_t slot = (h >> 24) & tmask;
         for (;;) {
             const unsigned long long e = table[slot];
@@ -247,7 +247,7 @@ __global__ void k_ht_probe(const UncSide* __restrict__ sides, const uint64_t* __
             if (e == kEmpty) break;
             if ((e >> 32) == (h & 0xFFFFFFFFull)) {
                 const uint32_t j = (uint32_t)(e & 0xFFFFFFFFull);
-                if (hashes[j] == h && sides[surv[j] >> 40].group == SA.group) {
+                if ((hashes[j] >> 1) == h) {
                     const unsigned long long o = atomicAdd(&ctl->ncand, 1ull);
                     if ((int64_t)o < ccap) { cand_a[o] = (uint32_t)i; cand_b[o] = j; }
                 }
@@ -277,15 +277,17 @@ __global__ void __launch_bounds__(64) k_pair_check(const UncSide* __restrict__ s
         const int64_t i = i0 + h;
         const bool live = i < nc;
         int N = 0;
+        bool same = false;  // both rows from one input pair
         if (live) {
             const uint64_t rec = surv[q == 1 ? cand_b[i] : cand_a[i]];
             const UncSide& S = sides[rec >> 40];
             N = S.N;
-            if (q < 2) decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, wl[h][q]);
+            same = S.group == sides[surv[q == 1 ? cand_a[i] : cand_b[i]] >> 40].group;
+            if (q < 2 && same) decode_word(S, rec & ((1ull << 40) - 1), radix, tup, nullptr, wl[h][q]);
         }
         __syncthreads();
         bool ok = true;
-        if (live) {
+        if (live && same) {
             const int8_t* wa = wl[h][0];
             const int8_t* wb = wl[h][1];
             const int s = q + 1;
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(64) k_pair_check(const UncSide* __restrict__ s
             }
         }
         const uint64_t bad = __ballot(!ok);
-        if (live && q == 0) {
+        if (live && same && q == 0) {
             ++checks;
             if (((bad >> (32 * h)) & 0xFFFFFFFFull) == 0) {
                 const unsigned long long o = atomicAdd(&ctl->out, 1ull);
@@ -1450,11 +1452,11 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ht_fill, dim3(ctx->num_cus * 16), dim3(256), 0, st, table, T - 1, F->jctl);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
-                       nsurv, F->rcap, F->jctl);
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, (int64_t)0, table, T - 1, nsurv, F->rcap,
+                       F->jctl);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, d_surv, hashes, (int64_t)0, table, T - 1,
-                       ca, cb, F->ccap, F->jctl, nsurv, F->rcap, true);
+    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, hashes, (int64_t)0, table, T - 1, ca, cb, F->ccap,
+                       F->jctl, nsurv, F->rcap, true);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pair_check, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
                        ca, cb, F->ccap, paf_target, q_target, oa, ob, F->ecap, F->jctl);
@@ -1559,8 +1561,8 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
                        paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, nullptr,
-                       (int64_t)0, nullptr);
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1, nullptr, (int64_t)0,
+                       nullptr);
     GP_HIP(hipGetLastError());
     // candidate pairs (the list is grown once if it overflows), then the exact
     // check, whose output cannot exceed the candidates
@@ -1570,8 +1572,8 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
         uint32_t* ca = ctx->dev<uint32_t>("u_ca", (size_t)ccap);
         uint32_t* cb = ctx->dev<uint32_t>("u_cb", (size_t)ccap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
-        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_surv, hashes, m, table, T - 1, ca,
-                           cb, ccap, ctl, nullptr, (int64_t)0, false);
+        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1, ca, cb, ccap, ctl,
+                           nullptr, (int64_t)0, false);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
