@@ -105,11 +105,11 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         UncTrig trig;
         if (P > 0) trig = unc_trig(ctx, N);
         const bool fp32 = ctx->psd_mode == GP_PSD_FP32;
-        auto prepare = [&, L](int64_t q0, int64_t q1) {
+        auto prepare = [&, L](int64_t q0, int64_t q1, const std::function<void()>& before_fill) {
             const std::span<const SideHost> sides(all + 2 * q0, (size_t)(2 * (q1 - q0)));
             const std::span<const int> grp(group.data(), sides.size());
             GP_HIP(hipSetDevice(ctx->device));  // the pinned staging buffer
-            return unc_prepare(sides, grp, order, paf, fp32, trig, &ctx->pinned);
+            return unc_prepare(sides, grp, order, paf, fp32, trig, &ctx->pinned, before_fill);
         };
         const auto tab = make_splits(compress, new_compress);
         auto build_range = [&](int64_t q0, int64_t q1) {
@@ -123,11 +123,16 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         std::vector<std::pair<int64_t, int64_t>> batches;
         using PrepFuture = std::shared_future<std::shared_ptr<UncPrep>>;
         PrepFuture next, after;
+        // a batch's splits and layout (mostly one thread) overlap the previous
+        // batch's table fill; its own fill (all host threads) waits for that one
         auto prepare_after = [&](PrepFuture prev, int64_t q0, int64_t q1) {
             return std::async(std::launch::async, [&prepare, prev, q0, q1]() mutable {
-                       if (prev.valid()) prev.wait();
-                       prev = PrepFuture();  // the previous batch's tables are released with its own future
-                       return prepare(q0, q1);
+                       auto P = prepare(q0, q1, [&prev] {
+                           if (prev.valid()) prev.wait();
+                           prev = PrepFuture();  // the previous batch's tables are released with its own future
+                       });
+                       prev = PrepFuture();
+                       return P;
                    }).share();
         };
         {

@@ -3,6 +3,7 @@
 
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <span>
@@ -84,11 +85,14 @@ UncTrig unc_trig(gp_ctx* ctx, int N);
 // unc_launch uploads the tables and runs the filter (unc_filter = both).
 // Every table is written in place into one staging buffer: a pinned buffer
 // from `pool` when given, else host memory of the batch (then unc_stage copies
-// it into a pinned buffer before the upload).
+// it into a pinned buffer before the upload).  `before_fill`, when given, runs
+// once the layout is known and before the tables are filled on the host
+// threads (a caller waits there for an earlier batch's fill).
 struct UncPrep;
 struct PinnedPool;
 std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
-                                     bool fp32, const UncTrig& trig, PinnedPool* pool = nullptr);
+                                     bool fp32, const UncTrig& trig, PinnedPool* pool = nullptr,
+                                     const std::function<void()>& before_fill = {});
 UncBatch unc_launch(gp_ctx* ctx, UncPrep& prep, gp_stats* stats);
 void unc_stage(UncPrep& prep, PinnedPool& pool);
 

@@ -962,7 +962,8 @@ UncTrig unc_trig(gp_ctx* ctx, int N) {
 }
 
 std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<const int> group, int order, int paf,
-                                     bool fp32, const UncTrig& trig, PinnedPool* pinned) {
+                                     bool fp32, const UncTrig& trig, PinnedPool* pinned,
+                                     const std::function<void()>& before_fill) {
     auto prep = std::make_shared<UncPrep>();
     UncPrep& U = *prep;
     if (sides.empty()) return prep;
@@ -1205,6 +1206,10 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         std::memset(T.tup, 0, bytes[kStTup]);
         std::memset(T.pm1, 0, bytes[kStPm1]);
         std::memset(T.fix32, 0, bytes[kStFix32]);
+        if (before_fill) {
+            ph1.next("tables: wait");
+            before_fill();
+        }
         ph1.next("tables: fill");
         // every table built in place on host threads: weight sets, mid and suffix
         // tables, then the per-side rows (which need the sets' BF16 residuals)
