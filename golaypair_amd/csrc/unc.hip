@@ -387,7 +387,8 @@ __global__ void __launch_bounds__(64) k_emit_rows(const UncSide* __restrict__ si
 // combination of the suffix positions' tuples (mixed-radix order, last
 // position fastest), laid out [tile][block][lane] as uint4 of 8 BF16 (entry
 // e = 16 block + 8 (lane / 32) + i of combination 32 tile + lane % 32, entry
-// e = x r + i being element i of suffix position x's tuple).  `side` is a
+// e = x r + i being element i of suffix position x's tuple).  The tile count
+// is rounded up to even (the filter loads tiles in pairs); padding is zero.  `side` is a
 // table side with these positions and letters (its tuple rows are used).
 struct SufJob {
     int64_t out_off;  // in uint4
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(256) k_build_suf(const SufJob* __restrict__ jo
     const int32_t* sp = pos + J.pos_off;
     int64_t ps = 1;
     for (int x = 0; x < J.npos; ++x) ps *= radix[S.radix_off + sp[x]];
-    const int64_t ntiles = (ps + 31) / 32;
+    const int64_t ntiles = (ps + 63) / 64 * 2;
     uint4* out = sufb + J.out_off;
     for (int64_t u = threadIdx.x; u < ntiles * nbs * 64; u += blockDim.x) {
         const int lane = (int)(u & 63), blk = (int)((u >> 6) % nbs);
@@ -607,7 +608,7 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, uint4* 
         return t;
     }();
     const int nbs = std::max(1, (nent + 15) / 16);
-    const int ntiles = (int)((ps + 31) / 32);
+    const int ntiles = (int)((ps + 63) / 64 * 2);  // even: tiles go in pairs
     for (int tile = 0; tile < ntiles; ++tile)
         for (int blk = 0; blk < nbs; ++blk)
             for (int lane = 0; lane < 64; ++lane) {
@@ -631,7 +632,7 @@ void build_suf(const SideHost& S, const std::vector<int>& sp, bool fp32, uint4* 
 int64_t mid_size(const Split& X, int mt) { return X.pm * mt * 32; }
 int64_t suf_size(const Split& X, bool fp32) {
     if (fp32) return X.ps * std::max(X.nent, 1);
-    return ((X.ps + 31) / 32) * std::max(1, (X.nent + 15) / 16) * 64;
+    return ((X.ps + 63) / 64 * 2) * std::max(1, (X.nent + 15) / 16) * 64;
 }
 
 // FP32 suffix DFT weights of the suffix positions, [nent][mt*32] (FP32 path).
