@@ -331,14 +331,48 @@ int filter_pairs(gp_ctx* ctx, int order, int compress, const PairSet& in, PairSe
         return false;
     };
     auto pair_less = [&](int64_t x, int64_t y) { return words_less(x, y, CW, RW); };
-    auto canon_less = [&](int64_t x, int64_t y) { return words_less(x, y, 0, CW); };
-    std::vector<int64_t> idx(m);
-    std::iota(idx.begin(), idx.end(), 0);
-    parallel_sort(idx, [&](int64_t x, int64_t y) { return words_less(x, y, 0, RW); });
+    auto canon_eq = [&](int64_t x, int64_t y) {
+        for (int w = 0; w < CW; ++w)
+            if (rk[x * RW + w] != rk[y * RW + w]) return false;
+        return true;
+    };
+    // Classes by hashing the canonical key, each keeping its least pair: rows go
+    // to 16 partitions by hash (a counting pass), each partition has its own
+    // open-addressing table on a host thread; the kept pairs are then sorted.
+    constexpr int kParts = 16;
+    std::vector<uint64_t> hsh((size_t)m);
+    parallel_chunks(m, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            uint64_t h = 0x9e3779b97f4a7c15ull;
+            for (int w = 0; w < CW; ++w) h = (h ^ rk[i * RW + w]) * 0xff51afd7ed558ccdull;
+            hsh[i] = h ^ (h >> 29);
+        }
+    }, 4096);
+    std::vector<int64_t> part_off(kParts + 1, 0), part_rows((size_t)m);
+    for (int64_t i = 0; i < m; ++i) ++part_off[(hsh[i] >> 60) + 1];
+    for (int q = 0; q < kParts; ++q) part_off[q + 1] += part_off[q];
+    {
+        std::vector<int64_t> fill(part_off.begin(), part_off.end() - 1);
+        for (int64_t i = 0; i < m; ++i) part_rows[fill[hsh[i] >> 60]++] = i;
+    }
+    std::vector<std::vector<int64_t>> part_keep(kParts);
+    pool().parallel_for(kParts, [&](size_t q) {
+        const int64_t r0 = part_off[q], r1 = part_off[q + 1];
+        size_t T = 16;
+        while (T < 2 * (size_t)(r1 - r0)) T <<= 1;
+        std::vector<int64_t> slot(T, -1);
+        for (int64_t r = r0; r < r1; ++r) {
+            const int64_t i = part_rows[r];
+            size_t s = hsh[i] & (T - 1);
+            while (slot[s] >= 0 && !canon_eq(slot[s], i)) s = (s + 1) & (T - 1);
+            if (slot[s] < 0 || pair_less(i, slot[s])) slot[s] = i;
+        }
+        for (size_t s = 0; s < T; ++s)
+            if (slot[s] >= 0) part_keep[q].push_back(slot[s]);
+    });
     std::vector<int64_t> keep;
-    for (int64_t r = 0; r < m; ++r)
-        if (r == 0 || canon_less(idx[r - 1], idx[r])) keep.push_back(idx[r]);
-    std::sort(keep.begin(), keep.end(), pair_less);
+    for (const auto& k : part_keep) keep.insert(keep.end(), k.begin(), k.end());
+    parallel_sort(keep, pair_less);
     out.a.resize(keep.size() * n);
     out.b.resize(keep.size() * n);
     for (size_t r = 0; r < keep.size(); ++r) {
