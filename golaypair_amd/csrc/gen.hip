@@ -30,10 +30,19 @@ namespace gp {
 
 typedef unsigned __int128 u128;
 
-struct GenSeg {       // one letter-index sum J
+// One segment: the words of letter-index sum J whose first letter d0 is their
+// least letter (the rest lie in d0..k-1; a necklace always has this form).
+// Its words are d0 followed by the n-1 letter suffixes d0 + s, s over the
+// alphabet 0..ks-1 (ks = k - d0) with index sum Js = J - n d0, ranked in
+// lexicographic order with count table cnt + cnt_off.
+struct GenSeg {
     int32_t J;
     int32_t side;     // bit 0: A candidates (AGL), bit 1: B candidates (dihedral)
-    int64_t count;    // words with index sum J
+    int64_t count;    // words of the segment
+    int32_t d0, ks;   // first letter; suffix alphabet size
+    int32_t Js;       // suffix index sum (shifted letters)
+    int32_t pad;
+    int64_t cnt_off;  // count table of alphabet size ks
 };
 struct GenTask {
     int32_t seg;
@@ -165,9 +174,16 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
         const GenSeg S = segs[T.seg];
         int64_t r0 = T.start + (int64_t)lane * T.per_lane;
         int64_t r1 = min(r0 + (int64_t)T.per_lane, T.end);
-        Wd w = 0;
+        // word = d0, then the suffix (n - 1 letters) with d0 added to each letter
+        const Wd head = (Wd)S.d0 << (4 * (n - 1));
+        Wd add = 0;
+        for (int p = 0; p < n - 1; ++p) add = (add << 4) | (Wd)S.d0;
+        Wd sfx = 0, w = 0;
         bool live = r0 < r1;
-        if (live) w = unrank<Wd>(r0, S.J, n, k, cnt, cstride);
+        if (live) {
+            sfx = unrank<Wd>(r0, S.Js, n - 1, S.ks, cnt + S.cnt_off, cstride);
+            w = head | (sfx + add);
+        }
         int64_t r = r0;
         // every lane steps until the whole wave is done
         while (__any(live)) {
@@ -216,7 +232,8 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
             }
             if (live) {
                 ++r;
-                live = r < r1 && successor(w, n, k);
+                live = r < r1 && successor(sfx, n - 1, S.ks);
+                w = head | (sfx + add);
             }
         }
     }
@@ -339,27 +356,39 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         side_of_J[J] |= bit;
     };
     for (auto& d : decomps) { addJ(d.first, 1); addJ(d.second, 2); }
-    // counts cnt[m][j]
+    // counts cnt_a[m][j]: words of m letters over the alphabet 0..a-1 with index
+    // sum j, one table per alphabet size a = 1..k (table a at (a - 1) * tsize)
     const int jmax = n * (k - 1);
     const int cstride = jmax + 1;
-    std::vector<uint64_t> cnt((size_t)(n + 1) * cstride, 0);
-    cnt[0] = 1;
+    const size_t tsize = (size_t)(n + 1) * cstride;
+    std::vector<uint64_t> cnt(tsize * k, 0);
     const uint64_t cap = (uint64_t)1 << 62;
-    for (int m = 1; m <= n; ++m)
-        for (int j = 0; j <= jmax; ++j) {
-            unsigned __int128 s = 0;
-            for (int d = 0; d < k && d <= j; ++d) s += cnt[(m - 1) * cstride + (j - d)];
-            cnt[m * cstride + j] = s > cap ? cap : (uint64_t)s;
-        }
+    for (int a = 1; a <= k; ++a) {
+        uint64_t* c = &cnt[(a - 1) * tsize];
+        c[0] = 1;
+        for (int m = 1; m <= n; ++m)
+            for (int j = 0; j <= jmax; ++j) {
+                unsigned __int128 s = 0;
+                for (int d = 0; d < a && d <= j; ++d) s += c[(m - 1) * cstride + (j - d)];
+                c[m * cstride + j] = s > cap ? cap : (uint64_t)s;
+            }
+    }
+    // a kept word is <= its rotations, so its first letter is its least letter:
+    // one segment per (J, first letter d0) over the suffixes with letters >= d0
     std::vector<GenSeg> segs;
     std::vector<GenTask> tasks;
     int64_t total = 0;
-    for (auto& [J, side] : side_of_J) {
-        GenSeg s{J, side, (int64_t)cnt[n * cstride + J]};
-        if ((uint64_t)s.count >= cap) throw ArgError("gen: search space too large");
-        segs.push_back(s);
-        total += s.count;
-    }
+    for (auto& [J, side] : side_of_J)
+        for (int d0 = 0; d0 < k; ++d0) {
+            const int ks = k - d0, Js = J - n * d0;
+            if (Js < 0 || Js > (n - 1) * (ks - 1)) continue;
+            const size_t off = (size_t)(ks - 1) * tsize;
+            GenSeg s{J, side, (int64_t)cnt[off + (size_t)(n - 1) * cstride + Js], d0, ks, Js, 0, (int64_t)off};
+            if ((uint64_t)s.count >= cap) throw ArgError("gen: search space too large");
+            if (s.count == 0) continue;
+            segs.push_back(s);
+            total += s.count;
+        }
     // ranks per lane: about 8 wave tasks per resident wave (so that the last
     // round of tasks is short), at least 16 ranks to amortise each lane's unrank
     const int64_t target_tasks = (int64_t)ctx->num_cus * 32 * 8;
