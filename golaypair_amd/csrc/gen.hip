@@ -163,15 +163,59 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
                                                   const uint32_t* __restrict__ shard_bits,
                                                   u128* __restrict__ surv_w, uint8_t* __restrict__ surv_side,
                                                   GenCtl* ctl) {
+    // Necklaces (about 1 word in n) wait in a per-wave LDS queue; the group
+    // tests (dihedral, AGL) then run on 64 of them at once, so the lanes of a
+    // wave do not wait on the few that hold a necklace at each step.
+    __shared__ Wd queue_all[4][128];
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Wd* const queue = queue_all[wv];
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int n = P.n, k = P.k;
     const Wd mask = mask_n<Wd>(n);
+    const unsigned long long below = (1ull << lane) - 1ull;
     unsigned long long nwords = 0;
+    // group tests of queue[0 .. cnt) (cnt <= 64, one per lane) and the append of the kept words
+    auto drain = [&](int cnt, int side) {
+        const bool have = lane < cnt;
+        const Wd w = have ? queue[lane] : (Wd)0;
+        bool keepA = false, keepB = false;
+        if (have) {
+            bool dih = true;
+            for (int j = 0; j < P.nunits_b && dih; ++j)
+                dih = le_all_rotations(w, decimate(w, n, P.units_b[j]), n, mask, 0);
+            keepB = dih && (side & 2);
+            if (dih && (side & 1)) {
+                bool agl = true;
+                for (int j = 0; j < P.nunits_a && agl; ++j) {
+                    const int u = P.units_a[j];
+                    if (u == n - 1) continue;  // covered by the dihedral test
+                    agl = le_all_rotations(w, decimate(w, n, u), n, mask, 0);
+                }
+                keepA = agl;
+            }
+        }
+        const unsigned long long ba = __ballot(keepA), bb = __ballot(keepB);
+        const int tot = __popcll(ba) + __popcll(bb);
+        if (tot) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&ctl->survivors, (unsigned long long)tot);
+            base = __shfl(base, 0);
+            if (keepA) {
+                const unsigned long long i = base + __popcll(ba & below);
+                if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 0; }
+            }
+            if (keepB) {
+                const unsigned long long i = base + __popcll(ba) + __popcll(bb & below);
+                if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 1; }
+            }
+        }
+    };
     for (int64_t ti = wave; ti < ntasks; ti += nwaves) {
         const GenTask T = tasks[ti];
         const GenSeg S = segs[T.seg];
+        const int side = __builtin_amdgcn_readfirstlane(S.side);
         int64_t r0 = T.start + (int64_t)lane * T.per_lane;
         int64_t r1 = min(r0 + (int64_t)T.per_lane, T.end);
         // word = d0, then the suffix (n - 1 letters) with d0 added to each letter
@@ -185,9 +229,10 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
             w = head | (sfx + add);
         }
         int64_t r = r0;
+        int qn = 0;  // queued necklaces (wave-uniform, < 64 between steps)
         // every lane steps until the whole wave is done
         while (__any(live)) {
-            bool keepA = false, keepB = false;
+            bool neck = false;
             if (live) {
                 ++nwords;
                 bool ok = true;
@@ -197,38 +242,18 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
                     ok = (shard_bits[idx >> 5] >> (idx & 31)) & 1u;
                 }
                 // rotations (u = 1) first: the necklace condition
-                if (ok) ok = le_all_rotations(w, w, n, mask, 1);
-                if (ok) {
-                    bool dih = true;
-                    for (int j = 0; j < P.nunits_b && dih; ++j)
-                        dih = le_all_rotations(w, decimate(w, n, P.units_b[j]), n, mask, 0);
-                    keepB = dih && (S.side & 2);
-                    if (dih && (S.side & 1)) {
-                        bool agl = true;
-                        for (int j = 0; j < P.nunits_a && agl; ++j) {
-                            const int u = P.units_a[j];
-                            if (u == n - 1) continue;  // covered by the dihedral test
-                            agl = le_all_rotations(w, decimate(w, n, u), n, mask, 0);
-                        }
-                        keepA = agl;
-                    }
-                }
+                neck = ok && le_all_rotations(w, w, n, mask, 1);
             }
-            const unsigned long long ba = __ballot(keepA), bb = __ballot(keepB);
-            const int tot = __popcll(ba) + __popcll(bb);
-            if (tot) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->survivors, (unsigned long long)tot);
-                base = __shfl(base, 0);
-                const unsigned long long below = (1ull << lane) - 1ull;
-                if (keepA) {
-                    const unsigned long long i = base + __popcll(ba & below);
-                    if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 0; }
-                }
-                if (keepB) {
-                    const unsigned long long i = base + __popcll(ba) + __popcll(bb & below);
-                    if ((int64_t)i < P.capacity) { surv_w[i] = (u128)w; surv_side[i] = 1; }
-                }
+            const unsigned long long bn = __ballot(neck);
+            if (neck) queue[qn + __popcll(bn & below)] = w;
+            qn += __popcll(bn);
+            if (qn >= 64) {
+                drain(64, side);
+                // the rest (fewer than 64) moves to the front
+                const int rest = qn - 64;
+                const Wd t = lane < rest ? queue[64 + lane] : (Wd)0;
+                if (lane < rest) queue[lane] = t;
+                qn = rest;
             }
             if (live) {
                 ++r;
@@ -236,6 +261,7 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
                 w = head | (sfx + add);
             }
         }
+        if (qn > 0) drain(qn, side);
     }
     wave_atomic_add(&ctl->words, nwords);
 }
