@@ -126,6 +126,14 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
         int d = 0;
         if (R == 1) {
             // single-split position: digit 0, index unchanged
+        } else if (R == 3 && leaf < 0x100000000ull) {  // the ±1 letters of a 3-fold expansion
+            const uint32_t l32 = (uint32_t)leaf;
+            const uint32_t q = __umulhi(l32, 0xAAAAAAABu) >> 1;  // exact l32 / 3 for every 32-bit l32
+            d = (int)(l32 - 3u * q);
+            leaf = q;
+        } else if (R == 2) {
+            d = (int)(leaf & 1);
+            leaf >>= 1;
         } else if (leaf < 0x100000000ull) {
             const uint32_t l32 = (uint32_t)leaf;
             d = (int)(l32 % R);
