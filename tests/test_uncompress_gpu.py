@@ -154,6 +154,22 @@ def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs, m
         open(os.path.join(wd, "%d-unique" % order)).read()
 
 
+@pytest.mark.parametrize("seqflag", [0, 1])
+def test_uncompress_side_mid_rows_beyond_one_chunk(gpu_ctx, seqflag):
+    """A length-35 side expanded 5 -> 1 whose active positions all have 10
+    splits (position 0: 2 after the dihedral reduction): suffix 3 positions
+    (1000 leaves), mid 2 positions (100 rows), outer position 0.  The filter's
+    one-block one-tile class stages mid-row seeds 32 rows at a time, so this side
+    runs four chunks (32, 32, 32, 4) per work item.  DFS candidate lines vs the
+    oracle."""
+    seq = np.array([1, -1, 1, 5, 1, -1, -1], np.int8)
+    ref_text, ref_leaves = _oracle.uncompress_side(seq, 5, 1, -2, seqflag)
+    cands, rot, st = gpu_ctx.uncompress_side(seq, 5, 1, -2, seqflag)
+    assert ref_leaves == 200000 and st.psd_decisions == ref_leaves
+    assert len(ref_text) > 0
+    assert "".join(l + "\n" for l in cands.lines()) == ref_text
+
+
 def test_uncompress_side_v50_full_shape(gpu_ctx):
     """Single sequences of the v=50 5-compression level expanded 5 -> 1 (length 50:
     two M-tiles, fixed channels k = 5m dropped, single-split positions folded):
