@@ -182,7 +182,7 @@ struct gp_ctx {
     std::string err;
     gp::PsdTables tables;
     // scratch owned by the context
-    gp::DevBuf words, tiles, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
+    gp::DevBuf words, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     int num_cus = 256;
     // result of the last gp_uncompress that failed with GP_ERR_CAPACITY
@@ -215,8 +215,8 @@ double fp32_E(int n, double s1);
 FilterThr make_thr(double thr, double E);
 
 // Filters rows [0, nrows) of `d_words` (stride bytes per row, rows zero
-// padded to 16*ceil(n/16)).  `d_tiles` lists 32-row tiles of one table each
-// (split mode) and `segs` the same rows as per-table segments (FP32 mode).
+// padded to 16*ceil(n/16)).  `segs` lists the rows as per-table segments (at
+// most 8); the split kernel cuts them into 32-row tiles on the device.
 // Writes decisions (kReject/kAccept) after the FP64 recheck of band rows.
 // Returns PSD-kernel device time in ms; stats counts band rechecks and ties.
 struct BatchSeg {

@@ -177,8 +177,18 @@ __device__ __forceinline__ void psd_tile(const int8_t* __restrict__ words, int s
     }
 }
 
+// The 32-row tiles of up to 8 row segments, passed by value: tile t of the
+// batch is tile t - tile0[s] of the segment s with tile0[s] <= t < tile0[s + 1].
+struct SegTiles {
+    int32_t nseg;
+    int32_t table[8];
+    int64_t first_row[8];
+    int64_t rows[8];
+    int64_t tile0[9];
+};
+
 __global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ words, int stride,
-                                                  const TileDesc* __restrict__ tiles, int64_t ntiles,
+                                                  const SegTiles T, int64_t ntiles,
                                                   const PsdTableDesc* __restrict__ tabs,
                                                   const uint4* __restrict__ frags,
                                                   const FilterThr* __restrict__ thr,
@@ -188,7 +198,10 @@ __global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ wor
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t ti = wave; ti < ntiles; ti += nwaves) {
-        const TileDesc td = tiles[ti];
+        int sg = 0;
+        while (sg + 1 < T.nseg && ti >= T.tile0[sg + 1]) ++sg;
+        const int64_t r0 = (ti - T.tile0[sg]) * 32;
+        const TileDesc td{T.table[sg], (int32_t)min<int64_t>(32, T.rows[sg] - r0), T.first_row[sg] + r0};
         const PsdTableDesc d = tabs[td.table];
         const uint4* af = frags + d.frag_off;
         const FilterThr th = thr[td.table];
@@ -293,27 +306,32 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
     if (segs.size() > 8) throw ArgError("at most 8 word lengths per PSD batch");
     hipStream_t st = ctx->stream;
     ctx->tables.upload(st);
-    // tiles
-    std::vector<TileDesc> tiles;
-    for (const BatchSeg& s : segs)
-        for (int64_t r = 0; r < s.rows; r += 32)
-            tiles.push_back(TileDesc{s.table, (int32_t)std::min<int64_t>(32, s.rows - r), s.first_row + r});
+    // tiles: computed on the device from the segments
+    SegTiles T{};
+    T.nseg = (int32_t)segs.size();
+    int64_t ntiles = 0;
+    for (size_t i = 0; i < segs.size(); ++i) {
+        T.table[i] = segs[i].table;
+        T.first_row[i] = segs[i].first_row;
+        T.rows[i] = segs[i].rows;
+        T.tile0[i] = ntiles;
+        ntiles += (segs[i].rows + 31) / 32;
+    }
+    T.tile0[segs.size()] = ntiles;
     BandCtl* ctl = static_cast<BandCtl*>(ctx->counters.ensure(sizeof(BandCtl)));
     int64_t* band = static_cast<int64_t*>(ctx->band_rows.ensure(std::max<int64_t>(nrows, 1) * sizeof(int64_t)));
-    TileDesc* d_tiles = static_cast<TileDesc*>(ctx->tiles.ensure(std::max<size_t>(tiles.size(), 1) * sizeof(TileDesc)));
     FilterThr* d_thr = static_cast<FilterThr*>(ctx->thr.ensure(thr_by_table.size() * sizeof(FilterThr) + 64));
     BatchSeg* d_segs = static_cast<BatchSeg*>(ctx->aux5.ensure(segs.size() * sizeof(BatchSeg) + 64));
-    GP_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
     GP_HIP(hipMemcpyAsync(d_thr, thr_by_table.data(), thr_by_table.size() * sizeof(FilterThr), hipMemcpyHostToDevice, st));
     GP_HIP(hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, st));
     GP_HIP(hipMemsetAsync(ctl, 0, sizeof(BandCtl), st));
     GP_HIP(hipEventRecord(ctx->ev0, st));
     if (ctx->psd_mode == GP_PSD_SPLIT_BF16) {
-        int64_t waves = std::min<int64_t>((int64_t)tiles.size(), (int64_t)ctx->num_cus * 16);
+        int64_t waves = std::min<int64_t>(ntiles, (int64_t)ctx->num_cus * 16);
         int blocks = (int)std::max<int64_t>(1, (waves + 3) / 4);
-        if (!tiles.empty())
-            hipLaunchKernelGGL(k_psd_mfma, dim3(blocks), dim3(256), 0, st, d_words, stride, d_tiles,
-                               (int64_t)tiles.size(), ctx->tables.d_desc.as<PsdTableDesc>(),
+        if (ntiles > 0)
+            hipLaunchKernelGGL(k_psd_mfma, dim3(blocks), dim3(256), 0, st, d_words, stride, T, ntiles,
+                               ctx->tables.d_desc.as<PsdTableDesc>(),
                                ctx->tables.d_frags.as<uint4>(), d_thr, d_decision, ctl, band, d_maxpsd);
     } else {
         int blocks = (int)std::min<int64_t>((nrows + 255) / 256, (int64_t)ctx->num_cus * 8);
