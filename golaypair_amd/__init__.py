@@ -74,7 +74,8 @@ SIGNATURES = {
     "gp_match": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(GpCands),
                                 _P(GpCands), _P(GpPairs), _P(GpStats)]),
     "gp_uncompress_side": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int8), ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(GpCands), _P(GpStats)]),
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _P(GpCands), _P(GpStats)]),
     "gp_uncompress": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _P(GpPairs), _P(GpPairs), _P(GpCands), _P(GpCands), _P(GpStats)]),
     "gp_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(GpPairs), _P(GpPairs),
@@ -290,8 +291,9 @@ def _ctx_methods():
                                         ptr(keys, ctypes.c_uint8), None), "gp_psd_keys")
         return keys
 
-    def uncompress_side(self, orig, compress, new_compress, paf, seqflag):
-        """Candidates of one sequence in DFS order; returns (cands, rotated orig, stats)."""
+    def uncompress_side(self, orig, compress, new_compress, paf, seqflag, proc_id=0, proc_num=1):
+        """Candidates of one sequence (job proc_id of proc_num) in DFS order;
+        returns (cands, rotated orig, stats)."""
         o = np.ascontiguousarray(orig, dtype=np.int8).copy()
         N = len(o) * (compress // new_compress)
         cap = 1 << 12
@@ -301,7 +303,7 @@ def _ctx_methods():
             st = GpStats()
             o2 = o.copy()
             rc = self.lib.gp_uncompress_side(self.h, ptr(o2, ctypes.c_int8), len(o2), compress, new_compress, paf,
-                                             seqflag, ctypes.byref(s), ctypes.byref(st))
+                                             proc_id, proc_num, seqflag, ctypes.byref(s), ctypes.byref(st))
             if rc == GP_ERR_CAPACITY:
                 cap = s.count + 1
                 continue

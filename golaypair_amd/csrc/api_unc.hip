@@ -30,20 +30,33 @@ constexpr uint64_t kLeafBudget = 1ull << 30;  // leaves per device batch
 
 extern "C" {
 
-int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new_compress, int paf, int seqflag,
-                       gp_cands* out, gp_stats* stats) {
+int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new_compress, int paf, int proc_id,
+                       int proc_num, int seqflag, gp_cands* out, gp_stats* stats) {
     if (!ctx) return GP_ERR_ARG;
     return guarded_call(ctx, [&] {
         GP_HIP(hipSetDevice(ctx->device));
         if (!orig || len <= 0 || compress <= 1 || new_compress <= 0 || compress % new_compress || !out)
             throw ArgError("gp_uncompress_side: bad arguments");
+        if (proc_num < 1) throw ArgError("gp_uncompress_side: proc_num must be at least 1");
         const auto tab = make_splits(compress, new_compress);
         const SideHost S = build_side(*tab, orig, len, seqflag);
         for (int i = 0; i < len; ++i) orig[i] = (int8_t)S.orig[i];  // the reference rotates in place
+        // uncompression.cpp:167-191: the job split depth is the first number of
+        // leading positions whose partial words reach 1000 * proc_num (or all of
+        // them); job proc_id takes the partials of index proc_id mod proc_num.
+        // The device filters the whole side; the job's leaves are selected from
+        // its survivors by that index.
+        uint64_t job_suffix = 1;
+        if (proc_num > 1) {
+            uint64_t partials = 1;
+            int depth = 0;
+            while (depth < len && partials < 1000ull * (uint64_t)proc_num) partials *= (uint64_t)S.R[depth++];
+            for (int j = depth; j < len; ++j) job_suffix *= (uint64_t)S.R[j];
+        }
         const int order = len * compress;
         const int g0 = 0;
         UncBatch B = unc_filter(ctx, std::span<const SideHost>(&S, 1), std::span<const int>(&g0, 1), order, paf, stats);
-        CandSet c = unc_side_cands(ctx, B, 0);
+        CandSet c = unc_side_cands(ctx, B, 0, job_suffix, proc_id, proc_num);
         c.n = S.N;
         c.nkey = S.N / 2;
         return export_cands(c, out);

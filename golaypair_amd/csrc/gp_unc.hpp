@@ -112,6 +112,10 @@ struct UncInflight;
 std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> prep, int slot, int paf_target);
 UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E, gp_stats* stats);
 // Survivors of one input side in DFS order with FP64 keys.
-CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side);
+// Candidates of one input side in DFS order.  With proc_num > 1 only the
+// leaves of job proc_id are kept: those whose index divided by `job_suffix`
+// (the leaf count below the job-split depth) is proc_id mod proc_num.
+CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side, uint64_t job_suffix = 1, int proc_id = 0,
+                       int proc_num = 1);
 
 }  // namespace gp

@@ -1641,7 +1641,7 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     return E;
 }
 
-CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side) {
+CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side, uint64_t job_suffix, int proc_id, int proc_num) {
     CandSet c;
     c.n = B.N;
     c.nkey = B.K;
@@ -1663,6 +1663,13 @@ CandSet unc_side_cands(gp_ctx* ctx, const UncBatch& B, int input_side) {
     if (m) GP_HIP(hipMemcpyAsync(recs.data(), sel, m * 8, hipMemcpyDeviceToHost, st));
     GP_HIP(hipStreamSynchronize(st));
     std::sort(recs.begin(), recs.end());  // DFS order
+    if (proc_num > 1) {
+        const uint64_t leaf_mask = ((uint64_t)1 << 40) - 1;
+        std::erase_if(recs, [&](uint64_t r) {
+            return ((r & leaf_mask) / job_suffix) % (uint64_t)proc_num != (uint64_t)proc_id;
+        });
+        m = recs.size();
+    }
     if (m == 0) return c;
     GP_HIP(hipMemcpyAsync(sel, recs.data(), m * 8, hipMemcpyHostToDevice, st));
     int8_t* w = ctx->dev<int8_t>("u_cw", (size_t)m * B.N);

@@ -118,10 +118,16 @@ int gp_match(gp_ctx* ctx, int order, int compress, int paf, const gp_cands* a_so
  *   std::ofstream& outfile, int seqflag)
  *   (src/core/uncompression/gpu/uncompress_gpu.h:8; CPU semantics
  *   uncompression.cpp:81-210).  `orig` (length len) is rotated in place as the
- *   reference does.  Candidates are written in the reference's DFS order;
- *   seqflag 0 -> inverted keys, 1 -> plain keys. */
+ *   reference does.  Candidates of job proc_id of proc_num are written in the
+ *   reference's DFS order (uncompression.cpp:167-191: with proc_num > 1 the
+ *   leading positions are expanded until their partial words reach
+ *   1000 * proc_num, and the job keeps the partials of index proc_id mod
+ *   proc_num; proc_id outside [0, proc_num) keeps nothing); seqflag 0 ->
+ *   inverted keys, 1 -> plain keys.  stats counts every leaf of the sequence
+ *   (the device filters them all).  proc_num < 1 is GP_ERR_ARG. */
 int gp_uncompress_side(gp_ctx* ctx, int8_t* orig, int len, int compress, int new_compress,
-                       int paf, int seqflag, gp_cands* out, gp_stats* stats);
+                       int paf, int proc_id, int proc_num, int seqflag, gp_cands* out,
+                       gp_stats* stats);
 
 /* Uncompression of every pair, re-filter and re-match, batched on the GPU.
  * Replaces int uncompress_pipeline(int order, int compress, int new_compress,

@@ -485,7 +485,8 @@ std::set<Seq> generate_uncompress(const Seq& seq) {  // equivalence.cpp:26-44
 }
 
 void uncompress_recursive(Seq& orig, int compress, int new_compress, int paf_c, int seqflag,
-                          std::vector<std::string>& lines, uint64_t* leaves) {  // uncompression.cpp:81-210
+                          std::vector<std::string>& lines, uint64_t* leaves, int proc_id,
+                          int proc_num) {  // uncompression.cpp:81-210
     const int order = (int)orig.size() * compress;
     const int len = order / compress;
     const int r = compress / new_compress;
@@ -519,27 +520,48 @@ void uncompress_recursive(Seq& orig, int compress, int new_compress, int paf_c, 
     const std::vector<Seq> newfirsta(perma.begin(), perma.end());
 
     const int bound = order * 2 - paf_c;
-    Seq seq(newlen, 0);
     std::vector<double> p(newlen);
     uint64_t nleaf = 0;
-    // :144-165 DFS, position 0 outermost; :195-207 leaf PSD filter.
-    std::function<void(size_t)> dfs = [&](size_t idx) {
-        if (idx >= orig.size()) {
+    // :144-165  positions [idx, stop) filled in DFS order, position 0 outermost;
+    // `at_leaf` runs on each word with positions [.., stop) filled.
+    std::function<void(Seq&, size_t, size_t, const std::function<void(const Seq&)>&)> fill =
+        [&](Seq& seq, size_t idx, size_t stop, const std::function<void(const Seq&)>& at_leaf) {
+            if (idx >= orig.size() || idx == stop) { at_leaf(seq); return; }
+            const std::vector<Seq>& opts = idx == 0 ? newfirsta : partitions.at(orig[idx]);
+            for (const Seq& perm : opts) {
+                for (int i = 0; i < r; i++) seq[idx + (size_t)len * i] = perm[i];
+                fill(seq, idx + 1, stop, at_leaf);
+            }
+        };
+    // :167-191  with proc_num > 1, partial words are expanded one position at a
+    // time until there are at least 1000 * proc_num of them (or every position is
+    // set); job proc_id keeps the partials whose index is proc_id mod proc_num.
+    std::vector<Seq> prev_depth{Seq(newlen, 0)};
+    size_t depth = 0;
+    if (proc_num > 1) {
+        long long count = 0;
+        for (; depth < orig.size() && count < 1000LL * proc_num; ++depth) {
+            count = 0;
+            std::vector<Seq> curr_depth;
+            for (Seq& partial : prev_depth)
+                fill(partial, depth, depth + 1, [&](const Seq& s) { curr_depth.push_back(s); ++count; });
+            prev_depth = std::move(curr_depth);
+        }
+    }
+    std::vector<Seq> partial_list;
+    for (size_t i = 0; i < prev_depth.size(); ++i)
+        if ((int)i % proc_num == proc_id) partial_list.push_back(prev_depth[i]);
+    // :193-207  leaf PSD filter and candidate line.
+    for (Seq& partial : partial_list) {
+        fill(partial, depth, orig.size(), [&](const Seq& seq) {
             ++nleaf;
             psd(seq.data(), newlen, p.data());
             if (psd_filter(p, order, paf_c)) {
                 if (seqflag) lines.push_back(seq_psd_line(seq, p));
                 else lines.push_back(seq_psd_line_invert(seq, p, bound));
             }
-            return;
-        }
-        const std::vector<Seq>& opts = idx == 0 ? newfirsta : partitions.at(orig[idx]);
-        for (const Seq& perm : opts) {
-            for (int i = 0; i < r; i++) seq[idx + (size_t)len * i] = perm[i];
-            dfs(idx + 1);
-        }
-    };
-    dfs(0);
+        });
+    }
     if (leaves) *leaves += nleaf;
 }
 

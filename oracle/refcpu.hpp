@@ -112,13 +112,13 @@ std::vector<Seq> get_combinations(int len, const std::vector<int>& alphabet);
 // equivalence.cpp:26-44  closure of one tuple under rotation and reversal.
 std::set<Seq> generate_uncompress(const Seq& seq);
 
-// uncompression.cpp:81-210 with PROC_ID=0, PROC_NUM=1 (as called at :65-66).
-// `orig` is rotated in place exactly as the reference does.  Each leaf that
-// passes the PSD filter appends one candidate line; `leaves` (if given)
-// counts every leaf examined.
+// uncompression.cpp:81-210 (uncompress_pipeline calls it with PROC_ID=0,
+// PROC_NUM=1 at :65-66).  `orig` is rotated in place exactly as the reference
+// does.  Each leaf of job proc_id of proc_num that passes the PSD filter
+// appends one candidate line; `leaves` (if given) counts every leaf examined.
 void uncompress_recursive(Seq& orig, int compress, int new_compress, int paf,
                           int seqflag, std::vector<std::string>& lines,
-                          uint64_t* leaves);
+                          uint64_t* leaves, int proc_id = 0, int proc_num = 1);
 
 // uncompression.cpp:32-79  pairs text at `compress` -> appended pairs text at
 // `new_compress`.  Returns 1 on a length mismatch, else 0.  Throws on a

@@ -153,14 +153,14 @@ int64_t ref_generate(int order, int compress, int paf, int job, int njobs, char*
     return rc;
 }
 
-// uncompression.cpp:81-210 on one sequence; returns the candidate-file text
-// size (lines in leaf DFS order); *leaves = leaves examined.
+// uncompression.cpp:81-210 on one sequence, job proc_id of proc_num; returns
+// the candidate-file text size (lines in leaf DFS order); *leaves = leaves examined.
 int64_t ref_uncompress_side(const int32_t* orig, int len, int compress, int new_compress, int paf, int seqflag,
-                            char* out, int64_t cap, uint64_t* leaves) {
+                            int proc_id, int proc_num, char* out, int64_t cap, uint64_t* leaves) {
     Seq o(orig, orig + len);
     std::vector<std::string> lines;
     uint64_t nl = 0;
-    refcpu::uncompress_recursive(o, compress, new_compress, paf, seqflag, lines, &nl);
+    refcpu::uncompress_recursive(o, compress, new_compress, paf, seqflag, lines, &nl, proc_id, proc_num);
     if (leaves) *leaves = nl;
     std::string t;
     for (auto& l : lines) { t += l; t += "\n"; }

@@ -53,7 +53,8 @@ def load():
                                                       ctypes.c_int64, _P(ctypes.c_int64)]
     lib.ref_uncompress_side.restype = ctypes.c_int64
     lib.ref_uncompress_side.argtypes = [_P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_char_p, ctypes.c_int64, _P(ctypes.c_uint64)]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64,
+                                        _P(ctypes.c_uint64)]
     lib.ref_tie_band.restype = ctypes.c_uint64
     _lib = lib
     return lib
@@ -134,14 +135,16 @@ def generate(order, c, paf, job=0, njobs=1):
     return rc, ba.value.decode(), bb.value.decode(), sizes[2]
 
 
-def uncompress_side(orig, c, c2, paf, seqflag):
+def uncompress_side(orig, c, c2, paf, seqflag, proc_id=0, proc_num=1):
     lib = load()
     o, po = i32(orig)
     leaves = ctypes.c_uint64()
     cap = 1 << 24  # one pass when the text fits; the call reports the size it needs
     buf = ctypes.create_string_buffer(cap)
-    n = lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, buf, cap, ctypes.byref(leaves))
+    n = lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, proc_id, proc_num, buf, cap, ctypes.byref(leaves))
     if n + 1 > cap:
+        o, po = i32(orig)
         buf = ctypes.create_string_buffer(n + 1)
-        lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, buf, n + 1, ctypes.byref(leaves))
+        lib.ref_uncompress_side(po, len(o), c, c2, paf, seqflag, proc_id, proc_num, buf, n + 1,
+                                ctypes.byref(leaves))
     return buf.value.decode(), leaves.value

@@ -139,3 +139,44 @@ def test_cli_exit_codes():
         assert run(g.CLI_PATH, [16, "-c", 2, "match"], d).returncode == 1   # no sorted files
         r = run(g.CLI_PATH, [16, "-c", 2, 1, "filter"], d)             # missing pairs file
         assert r.returncode == 0 and "Bad File." in r.stdout
+
+
+def _sha(path):
+    import hashlib
+    data = open(path, "rb").read()
+    return {"lines": data.count(b"\n"), "sha256": hashlib.sha256(data).hexdigest()}
+
+
+@pytest.mark.parametrize("fixture,args", [
+    ("level_v90_c10_5.json", [90, "--compress", 10, 5, "all"]),
+    ("level_v45_c3_1_paf-2.json", [45, "--paf", -2, "--compress", 3, 1, "all"]),
+])
+def test_cli_whole_level_matches_oracle_digests(fixture, args):
+    """Whole levels at full size: every matched pair of the coarse level is
+    uncompressed (v=90: 156 pairs, 1.9e8 leaves at length 18; v=45: 7 577 pairs,
+    2.9e9 leaves at length 45) and filtered.  The pairs files at both compressions
+    and the unique file must be the oracle's bytes (sha256 and line counts written
+    by tests/golden/make_golden.py)."""
+    import json
+    fx = json.load(open(os.path.join(GOLDEN, fixture)))
+    with tempfile.TemporaryDirectory() as d:
+        r = run(g.CLI_PATH, args, d)
+        assert r.returncode == 0, r.stderr
+        wd = os.path.join(d, "order-%d" % fx["order"])
+        for name, want in fx["files"].items():
+            assert _sha(os.path.join(wd, name)) == want, name
+
+
+@pytest.mark.parametrize("order,route,classes", [
+    (32, [2, 1], 838),          # NOTES.md:571-599 (838 after the altnegative fix, as in the 2022 paper)
+    (40, [4, 2, 1], 9281),      # NOTES.md:605-616 (the reference's count, with and without compression)
+])
+def test_cli_known_class_counts(order, route, classes):
+    """Equivalence-class counts the reference's own notes record for orders 32 and
+    40, through the whole GPU pipeline (gen, sort, match, uncompress to +-1, filter
+    with altnegative at length 32 / 40)."""
+    with tempfile.TemporaryDirectory() as d:
+        r = run(g.CLI_PATH, [order, "--compress"] + route + ["all"], d)
+        assert r.returncode == 0, r.stderr
+        got = open(os.path.join(d, "order-%d" % order, "%d-unique" % order)).read()
+        assert got.count("\n") == classes

@@ -112,3 +112,69 @@ def test_filter_alphabets_vs_oracle(gpu_ctx, n, compress, letters):
         ref = open(os.path.join(wd, "%d-unique" % order)).read()
     oa, ob = _gpu_filter(gpu_ctx, order, compress, a, b)
     assert _pairs_text(oa, ob) == ref
+
+
+def _lexmin_rows(m):
+    """Lexicographically smallest row of a 2-D integer array."""
+    order = np.lexsort(m.T[::-1])
+    return m[order[0]]
+
+
+def orbit_min_brute(a, b, alt):
+    """Minimum of the pair's orbit under the signed affine family, by brute force
+    over every group element (swap, u, e, kappa, sa, sb, ta, tb): for a fixed
+    (swap, u, e, kappa, sa, sb) the image's a-part depends only on ta and its b-part
+    only on tb, so the lexicographic minimum over (ta, tb) is (min over ta, min over tb)."""
+    n = len(a)
+    i = np.arange(n)
+    shifts = np.arange(n)[:, None]
+    best = None
+    for sw in (0, 1):
+        X, Y = (np.asarray(b), np.asarray(a)) if sw else (np.asarray(a), np.asarray(b))
+        for u in [u for u in range(1, n) if np.gcd(u, n) == 1]:
+            for e in (1, -1):
+                for kappa in ((0, 1) if alt else (0,)):
+                    sgn = np.where((i % 2 == 1) & (kappa == 1), -1, 1)
+                    imgs_a = X[(u * i + shifts) % n] * sgn          # row ta: the image's a-part
+                    imgs_b = Y[(e * u * i + shifts) % n] * sgn      # row tb: the image's b-part
+                    for sa in ((1, -1) if alt else (1,)):
+                        ma = tuple(_lexmin_rows(imgs_a * sa).tolist())
+                        for sb in ((1, -1) if alt else (1,)):
+                            cand = ma + tuple(_lexmin_rows(imgs_b * sb).tolist())
+                            if best is None or cand < best:
+                                best = cand
+    return best
+
+
+def brute_unique(a, b, alt):
+    """The lexicographically smallest input pair of each orbit, in (a, b) order."""
+    keep = {}
+    for x, y in zip(a, b):
+        key = orbit_min_brute(x, y, alt)
+        p = (tuple(int(v) for v in x), tuple(int(v) for v in y))
+        if key not in keep or p < keep[key]:
+            keep[key] = p
+    return sorted(keep.values())
+
+
+def test_filter_length50_altnegative_vs_brute_force(gpu_ctx):
+    """Length 50 with altnegative (order 50, compress 1, filter.cpp:83-87): the group
+    has 2 * 20 * 2 * 2 * 4 * 50^2 = 1.6e6 elements, beyond the oracle's closure.
+    Planted orbits (random +-1 pairs, five random images each, plus a symmetric
+    pair) against the brute-force orbit minimum over every group element."""
+    n = 50
+    rng = np.random.default_rng(50)
+    pairs = []
+    bases = [(tuple(int(x) for x in rng.choice([-1, 1], n)), tuple(int(x) for x in rng.choice([-1, 1], n)))
+             for _ in range(5)]
+    bases.append((tuple((-1) ** (i // 2) for i in range(n)), tuple((-1) ** (i // 5) for i in range(n))))
+    for p in bases:
+        for _ in range(5):
+            pairs.append(apply(_random_map(rng, n, True), *p))
+    rng.shuffle(pairs)
+    a = np.array([p[0] for p in pairs], np.int8)
+    b = np.array([p[1] for p in pairs], np.int8)
+    want = brute_unique(a, b, True)
+    assert len(want) == len(bases)
+    oa, ob = _gpu_filter(gpu_ctx, n, 1, a, b)
+    assert [(tuple(int(v) for v in x), tuple(int(v) for v in y)) for x, y in zip(oa, ob)] == want

@@ -281,6 +281,180 @@ def test_uncompress_expansions_compress_back(oracle, seq, c, c2):
     assert leaves > 0
 
 
+def split_list(letter, r, fine):
+    """Every r-tuple over the finer alphabet summing to `letter`: the nondecreasing
+    tuples in lexicographic order, each followed by its distinct permutations in
+    lexicographic order (the order in which the expansions are visited)."""
+    out = []
+    for comb in itertools.combinations_with_replacement(sorted(fine), r):
+        if sum(comb) == letter:
+            out += sorted(set(itertools.permutations(comb)))
+    return out
+
+
+def dihedral_orbits(tuples):
+    """Orbits of the dihedral group D_r (rotations, reversal) on a set of r-tuples."""
+    seen, orbits = set(), []
+    for t in tuples:
+        if t in seen:
+            continue
+        r = len(t)
+        orb = {tt for s in range(r) for tt in (t[s:] + t[:s], tuple(reversed(t[s:] + t[:s])))}
+        seen |= orb
+        orbits.append(orb)
+    return orbits
+
+
+def burnside_dihedral(tuples):
+    """Burnside count of the same orbits: the mean number of tuples fixed per group element."""
+    r = len(tuples[0])
+    fixed = 0
+    for s in range(r):
+        for rev in (False, True):
+            for t in tuples:
+                img = t[s:] + t[:s]
+                fixed += (tuple(reversed(img)) if rev else img) == t
+    assert fixed % (2 * r) == 0
+    return fixed // (2 * r)
+
+
+@pytest.mark.parametrize("seq,c,c2", [((3, -1, 1, -3, 1, 1, -1), 3, 1),
+                                      ((1, -1, 3, 1, -1, 1, -3, 3), 3, 1),
+                                      ((-1, 5, -3, 1, 1, -1), 5, 1),
+                                      ((1, 1, -3, 3, -1), 5, 1),
+                                      ((-2, 0, 2, 0, 0, 2, -2), 2, 1),
+                                      ((4, 0, -2, 2, 0, 6), 6, 2),
+                                      ((10, -2, 0, 4, -6, 2), 10, 5),
+                                      ((-2, 6, 0, 0, 2, -8, 4), 10, 5)])
+def test_uncompress_every_leaf_in_order(oracle, seq, c, c2):
+    """With a filter bound no word reaches (paf = -10^6) every leaf is written, so the
+    oracle's lines are the whole DFS (uncompression.cpp:81-210).  Checked against an
+    independent enumeration:
+      * the word is rotated so that it starts with the best letter, the smallest
+        letter with the most splits (:113-131), and it stays a rotation of the input;
+      * position 0 runs over one representative per dihedral orbit of its splits,
+        the lexicographically smallest, in increasing order (:133-142); the orbit
+        count equals Burnside's;
+      * every other position j runs over all splits of its letter, nondecreasing
+        tuples in lexicographic order, each with its permutations in lexicographic
+        order (:91-111); position 0 outermost;
+      * tuple element i lands at index j + L*i (:153,160);
+      * the leaf count is (orbits at position 0) x prod(split counts).
+    The rotation amount (which occurrence of the best letter ends up first) is the
+    one rule here that only follows the code."""
+    fine = alphabet(c2)
+    r, L = c // c2, len(seq)
+    text, leaves = _oracle.uncompress_side(seq, c, c2, -10 ** 6, 1)
+    words = [tuple(int(t) for t in line.split()[1:]) for line in text.splitlines()]
+    assert len(words) == leaves
+    rot = tuple(sum(words[0][j + L * i] for i in range(r)) for j in range(L))
+    assert rot in {tuple(seq[i:] + seq[:i]) for i in range(L)}
+    counts = {x: len(split_list(x, r, fine)) for x in set(seq)}
+    best = min(x for x in counts if counts[x] == max(counts.values()))
+    assert rot[0] == best
+    reps = sorted(min(o) for o in dihedral_orbits(split_list(best, r, fine)))
+    assert len(reps) == burnside_dihedral(split_list(best, r, fine))
+    opts = [reps] + [split_list(rot[j], r, fine) for j in range(1, L)]
+    want = []
+    for choice in itertools.product(*opts):
+        w = [0] * (L * r)
+        for j, t in enumerate(choice):
+            for i in range(r):
+                w[j + L * i] = t[i]
+        want.append(tuple(w))
+    assert words == want
+    assert leaves == len(reps) * math.prod(len(o) for o in opts[1:])
+
+
+@pytest.mark.parametrize("seq,c,c2,njobs", [((1, -1, 1, 5, 1, -1, -1), 5, 1, 3),
+                                            ((3, -1, 1, -3, 1, 1, -1, 1, -1), 3, 1, 2),
+                                            ((10, -2, 0, 4, -6, 2), 10, 5, 4),
+                                            ((1, 1, -3, 3, -1), 5, 1, 7)])
+def test_uncompress_job_split(oracle, seq, c, c2, njobs):
+    """uncompression.cpp:167-191: with PROC_NUM > 1 the partial words are expanded
+    position by position until there are >= 1000 * PROC_NUM of them (or all
+    positions are set), and job j keeps the partials of index j mod PROC_NUM.  With
+    every leaf written (paf = -10^6): the jobs partition the single-job lines, each
+    job's lines keep the DFS order, and job j holds exactly the leaves whose prefix
+    index at that depth is j mod PROC_NUM.  The depth is read off the radices."""
+    fine = alphabet(c2)
+    r, L = c // c2, len(seq)
+    text, leaves = _oracle.uncompress_side(seq, c, c2, -10 ** 6, 1)
+    whole = text.splitlines()
+    assert len(whole) == leaves
+    words = [tuple(int(t) for t in line.split()[1:]) for line in whole]
+    rot = tuple(sum(words[0][j + L * i] for i in range(r)) for j in range(L))
+    counts = {x: len(split_list(x, r, fine)) for x in set(seq)}
+    radix = [len(dihedral_orbits(split_list(rot[0], r, fine)))] + [counts[rot[j]] for j in range(1, L)]
+    depth, prod = 0, 1
+    while depth < L and prod < 1000 * njobs:
+        prod *= radix[depth]
+        depth += 1
+    suffix = math.prod(radix[depth:])
+    seen = []
+    for j in range(njobs):
+        t, lv = _oracle.uncompress_side(seq, c, c2, -10 ** 6, 1, j, njobs)
+        mine = t.splitlines()
+        assert lv == len(mine)
+        assert mine == [l for i, l in enumerate(whole) if (i // suffix) % njobs == j]
+        seen += mine
+    assert sorted(seen) == sorted(whole)
+    # the pipeline's own calls are single-job (uncompression.cpp:65-66)
+    assert _oracle.uncompress_side(seq, c, c2, 0, 0, 0, 1) == _oracle.uncompress_side(seq, c, c2, 0, 0)
+
+
+def is_prenecklace(w):
+    """Each suffix is >= the prefix of the same length (a prefix of some necklace)."""
+    return all(w[j:] >= w[:len(w) - j] for j in range(1, len(w)))
+
+
+def test_prenecklace_predicate_is_prefix_of_necklace():
+    # the predicate against its definition on a small alphabet: the length-i prefixes of
+    # the necklaces of lengths i..2i
+    for k, i in ((2, 5), (3, 4)):
+        neck = set()
+        for m in range(i, 2 * i + 1):
+            for w in itertools.product(range(k), repeat=m):
+                if all(w[s:] + w[:s] >= w for s in range(m)):
+                    neck.add(w[:i])
+        assert neck == {w for w in itertools.product(range(k), repeat=i) if is_prenecklace(w)}
+
+
+@pytest.mark.parametrize("order,c,lam,njobs", [(40, 5, 0, 2), (40, 5, 0, 3)])
+def test_job_split_assigns_prefix_rank_mod_jobs(oracle, order, c, lam, njobs):
+    """lyndon.cpp:87-121: the prefix length i is the least with necklace_count(i, k) >=
+    10^4 * jobs; the prenecklaces of length i are counted in lexicographic order from
+    1 and job j keeps those whose count % jobs == j.  Every line of job j must start
+    with a prefix of that rank."""
+    n, alpha = order // c, alphabet(c)
+    k = len(alpha)
+    i = next(m for m in range(1, n + 1)
+             if sum(k ** math.gcd(t, m) for t in range(1, m + 1)) // m >= 10 ** 4 * njobs)
+    assert i < n
+    rank, cnt = {}, 0
+    for w in itertools.product(alpha, repeat=i):     # lexicographic order
+        if is_prenecklace(w):
+            cnt += 1
+            rank[w] = cnt
+    total = 0
+    for j in range(njobs):
+        _, ta, tb, _ = _oracle.generate(order, c, lam, j, njobs)
+        for line in (ta + tb).splitlines():
+            w = tuple(int(t) for t in line.split()[1:])
+            assert rank[w[:i]] % njobs == j
+            total += 1
+    _, ta, tb, _ = _oracle.generate(order, c, lam)
+    assert total == len((ta + tb).splitlines()) > 0
+
+
+def test_job_split_without_enough_prefixes_repeats_the_whole_set(oracle):
+    """No prefix length reaches 10^4 * jobs necklaces below the word length, so the
+    wrapper takes the single-job branch in every job (lyndon.cpp:95,107-109)."""
+    whole = _oracle.generate(16, 2, 0)[1:3]
+    for j in range(2):
+        assert _oracle.generate(16, 2, 0, j, 2)[1:3] == whole
+
+
 # ---------------------------------------------------------------- known answers
 
 def test_notes_fixtures(oracle):

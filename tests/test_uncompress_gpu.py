@@ -114,7 +114,7 @@ def test_uncompress_compress1_is_noop(gpu_ctx):
     (90, [10, 5], 0, 3, None, True), (45, [3, 1], -2, 2, None, True)])
 def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs, max_leaves, with_filter):
     """Matched pairs of a full-size level (v=90 c=10 -> 5, filtered at length 18;
-    v=45 c=3 -> 1, length 45; v=50 c=5 -> 1, length 50): GPU gen/sort/match, the
+    v=45 c=3 -> 1, length 45): GPU gen/sort/match, the
     first `npairs` input pairs that emit pairs at the finer level (found one pair
     at a time on the GPU, at most `max_leaves` leaves each) plus an input pair
     that emits nothing, then uncompress (and the equivalence filter) on the GPU
@@ -147,7 +147,7 @@ def test_uncompress_slice_at_full_shape(gpu_ctx, order, compress, paf, npairs, m
     assert st.tie_band == 0 and len(oa) > 0
     assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)) == \
         open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
-    if not with_filter:  # the group at length 50 with altnegative is beyond the oracle's closure
+    if not with_filter:
         return
     fa, fb, _ = gpu_ctx.filter(order, c2, oa, ob)
     assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(fa, fb)) == \
@@ -188,3 +188,61 @@ def test_uncompress_side_v50_full_shape(gpu_ctx):
         assert st.psd_decisions == ref_leaves
         total += len(cands)
     assert total > 0  # some expansions pass the filter
+
+
+def test_uncompress_v50_pair_slice_vs_oracle_fixture(gpu_ctx):
+    """Config 4's own level (v=50, 5 -> 1, length 50, two M-tiles): the matched c=5
+    pairs with the fewest expansion leaves (about 10^8 leaves each, written with the
+    oracle's output by tests/golden/make_golden.py slice50).  The GPU's 50-pairs-1
+    lines for those input pairs must be the oracle's, in order; the input lines must
+    be the GPU's own match output at those positions; and the equivalence filter at
+    length 50 with altnegative must keep the brute-force survivors."""
+    import json
+    from tests.test_filter_gpu import brute_unique
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "slice_v50_c5_1.json")))
+    order, (c, c2), paf = fx["order"], fx["compress"], fx["paf"]
+    L = order // c
+    rows = [list(map(int, l.split())) for l in fx["pairs_in"].splitlines()]
+    pa = np.array([r[:L] for r in rows], np.int8)
+    pb = np.array([r[L:] for r in rows], np.int8)
+    a, b, _ = gpu_ctx.generate(order, c, paf)
+    rc, ma, mb, _ = gpu_ctx.match(order, c, paf, gpu_ctx.sort_candidates(a), gpu_ctx.sort_candidates(b))
+    assert rc == 0
+    for k, i in enumerate(fx["input_lines"]):
+        assert g.pair_line(ma[i], mb[i]) == fx["pairs_in"].splitlines()[k]
+    oa, ob, _, _, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb)
+    assert st.tie_band == 0
+    assert st.psd_decisions == sum(fx["leaves"])
+    assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)) == fx["pairs_out"]
+    assert len(oa) > 0
+    sel = slice(0, 40)  # the brute force takes about 0.2 s per pair
+    fa, fb, _ = gpu_ctx.filter(order, c2, oa[sel], ob[sel])
+    got = [(tuple(int(v) for v in x), tuple(int(v) for v in y)) for x, y in zip(fa, fb)]
+    assert got == brute_unique(oa[sel], ob[sel], True)
+
+
+@pytest.mark.parametrize("seq,c,c2,paf,njobs", [
+    ((1, -1, 1, 5, 1, -1, -1), 5, 1, -2, 3),             # 2*10^5 leaves, split at depth 5 of 7
+    ((3, -1, 1, -3, 1, 1, -1, 1, -1, 1), 3, 1, -2, 2),   # split at depth 8 of 10
+    ((10, -2, 0, 4, -6, 2), 10, 5, 0, 4),                # 1800 leaves < 4000: every position set
+    ((1, 1, -3, 3, -1), 5, 1, 0, 7),                     # 5000 leaves < 7000: every position set
+])
+def test_uncompress_side_job_split_matches_oracle(gpu_ctx, seq, c, c2, paf, njobs):
+    """The uncompress_gpu seam's PROC_ID / PROC_NUM (uncompress_gpu.h:8,
+    uncompression.cpp:167-191): each job's candidate lines in DFS order are the
+    oracle's, and the jobs together give the single-job lines."""
+    seq = np.array(seq, np.int8)
+    whole, _ = _oracle.uncompress_side(seq, c, c2, paf, 1)
+    parts = []
+    for j in range(njobs):
+        for seqflag in (0, 1):
+            ref_text, _ = _oracle.uncompress_side(seq, c, c2, paf, seqflag, j, njobs)
+            cands, rot, st = gpu_ctx.uncompress_side(seq, c, c2, paf, seqflag, j, njobs)
+            got = "".join(l + "\n" for l in cands.lines())
+            assert got == ref_text, (j, seqflag)
+            if seqflag:
+                parts += got.splitlines()
+    assert sorted(parts) == sorted(whole.splitlines())
+    assert len(whole) > 0
+    cands, _, _ = gpu_ctx.uncompress_side(seq, c, c2, paf, 1, njobs, njobs)   # no job of that id
+    assert len(cands) == 0
