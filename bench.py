@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
-"""Benchmark: PSD-filtered candidates/s of the v=45 Legendre-pair search on one MI355X.
+"""Benchmark: PSD-filtered candidates/s of the v=50 Golay-pair search on one MI355X.
 
-Workload (one step): the whole search of order v=45, PAF constant -2,
-compression 3 -> 1, on the GPU through the C ABI:
-  gen at c=3 (length-15 words over {-3,-1,1,3}, row sum 1, canonical, PSD filter),
-  sort + match at c=3, uncompression 3 -> 1 of every matched pair (length-45
-  words over {-1,1}, PSD filter + keys + match with exact PAF), equivalence filter.
+Workload (one step): the exhaustive search of order v=50, PAF constant 0,
+compression 5 -> 1, on the GPU through the C ABI:
+  gen at c=5 (length-10 words over {-5,-3,-1,1,3,5}, row sums (0, 10) and (6, 8),
+  canonical, PSD filter), sort + match at c=5 (4 691 pairs), uncompression 5 -> 1
+  of every matched pair (9.2e11 length-50 words over {-1,1}: PSD filter + keys +
+  re-match with exact PAF), equivalence filter at length 50 with altnegative.
 value = PSD-filter decisions of the step / wall time of the step (split-BF16 MFMA
-path); the plain FP32 path and the CPU oracle are reported beside it.
+path).  Beside it: the plain FP32 path on a fixed slice of the v=50 pairs (same
+output required), the mixed-length PSD microbenchmark, and the CPU oracle on P
+processes.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -15,8 +18,8 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
-import threading
 import time
 
 import numpy as np
@@ -24,9 +27,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-ORDER, C0, C1, PAF = 45, 3, 1, -2
-FLOPS_PER_LEAF = 4 * 45 * 22 + 3 * 22       # algorithmic: real DFT k=1..22 of n=45 + |X|^2
-MFMA_PEAK_TFLOPS = 2500.0                    # MI355X dense BF16
+ORDER, C0, C1, PAF = 50, 5, 1, 0
+N1 = ORDER // C1
+FLOPS_PER_LEAF = 4 * N1 * (N1 // 2) + 3 * (N1 // 2)   # algorithmic: real DFT k=1..25 of n=50 + |X|^2 = 5075
+MFMA_PEAK_TFLOPS = 2500.0                             # MI355X dense BF16
+FP32_SLICE = (0, 64)                                  # matched c=5 pairs [0, 64) in file order
+PROFILE = "k_unc_filter_v50_golay_search.md"
 
 
 STAGE_MS = {}
@@ -57,8 +63,9 @@ def search_step(ctx, g):
 
 def profiled_traffic(kernel="k_unc_filter"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (KiB per dispatch, scaled by a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "k_unc_filter_v45_legendre_search.md")
+    WRITE_SIZE summary of this workload (KiB per dispatch, scaled by what the same
+    counters read for a plain copy of known size).  Not measured by this run."""
+    path = os.path.join(ROOT, "profiles", PROFILE)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -81,34 +88,65 @@ def timed(ctx, g, steps, warmup):
     return (t1 - t0) / steps, out
 
 
-def cpu_baseline(pa, pb, seconds_hint=20.0):
-    """The oracle (direct FP64 DFT per leaf, one sequence per thread) on a bounded
-    sample: the uncompression of the first matched pairs, both sides."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_worker(argv):
+    """One single-threaded oracle process: job j of P of one side's uncompression
+    (direct FP64 DFT per leaf, PSD filter, candidate line), timed."""
     from tests import _oracle
+    seq = [int(x) for x in argv[0].split(",")]
+    seqflag, job, njobs = int(argv[1]), int(argv[2]), int(argv[3])
     _oracle.load()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    sides = []
-    for q in range(len(pa)):
-        sides.append((pa[q], 0))
-        sides.append((pb[q], 1))
-        if len(sides) >= threads:
-            break
-    leaves = [0] * len(sides)
-
-    def work(i):
-        _, leaves[i] = _oracle.uncompress_side(sides[i][0], C0, C1, PAF, sides[i][1])
-
     t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(i,)) for i in range(len(sides))]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    return {"value": sum(leaves) / dt, "unit": "candidates/s", "cores": len(sides), "kind": "port",
-            "sample": "oracle uncompression 3->1 of the first %d sequences of the v=45 matched pairs "
-                      "(%d leaves, %.1f s wall, one sequence per thread; direct FP64 DFT, FFTW absent)"
-                      % (len(sides), sum(leaves), dt)}
+    text, leaves = _oracle.uncompress_side(seq, C0, C1, PAF, seqflag, job, njobs)
+    print(json.dumps({"leaves": leaves, "seconds": time.perf_counter() - t0}))
+
+
+def side_leaves(w):
+    """Leaves of one c=5 side expanded to +-1: 10 splits of +-1, 5 of +-3, 1 of +-5,
+    the first (best) letter's splits counted once per dihedral orbit (2, 1, 1)."""
+    split = {1: 10, -1: 10, 3: 5, -3: 5, 5: 1, -5: 1}
+    first = {1: 2, -1: 2, 3: 1, -3: 1, 5: 1, -5: 1}
+    best = min(w, key=lambda x: (-split[int(x)], int(x)))
+    n = first[int(best)]
+    rest = [int(x) for x in w]
+    rest.remove(int(best))
+    for x in rest:
+        n *= split[x]
+    return n
+
+
+def cpu_baseline(pa, pb):
+    """The oracle on P single-threaded processes (P = the host cores this process
+    may use, at most 16): each takes job 0 of 2 of one of the P cheapest sides of
+    the v=50 matched pairs (uncompression.cpp:167-191 split), about 10-30 s each."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    P = max(1, min(16, cores))
+    sides = sorted({(side_leaves(w), tuple(int(x) for x in w), f) for ws, f in ((pa, 0), (pb, 1)) for w in ws})[:P]
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               ",".join(str(x) for x in w), str(f), "0", "2"], stdout=subprocess.PIPE, text=True)
+             for _, w, f in sides]
+    res = [json.loads(p.communicate()[0]) for p in procs]
+    wall = time.perf_counter() - t0
+    leaves = sum(r["leaves"] for r in res)
+    return {"value": leaves / wall, "unit": "candidates/s", "cores": P, "kind": "port",
+            "cpu_model": cpu_model(), "processes": P,
+            "sample": "oracle uncompression 5->1 of the %d cheapest sides of the v=50 matched pairs, job 0 of 2 "
+                      "each (uncompression.cpp:167-191), one single-threaded process per side: %d leaves in "
+                      "%.1f s wall (slowest process %.1f s); direct FP64 DFT per leaf, FFTW absent"
+                      % (P, leaves, wall, max(r["seconds"] for r in res))}
 
 
 def microbench(ctx, g, mode):
@@ -215,6 +253,9 @@ def main():
                     help="only: end-to-end CLI search time of %s (default all)" % ",".join(E2E_CONFIGS))
     ap.add_argument("--fp32-parity", type=int, metavar="ORDER", choices=(45, 50, 90),
                     help="only: uncompression of a whole level on both PSD paths, identical output and time ratio")
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(sys.argv[2:])
+        return
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -232,7 +273,7 @@ def main():
         sys.exit(2)
     import golaypair_amd as g
     ctx = g.Context(0)
-    sec, (st, su, npairs3, npairs1, nunique) = timed(ctx, g, args.steps, args.warmup)
+    sec, (st, su, npairs0, npairs1, nunique) = timed(ctx, g, args.steps, args.warmup)
     value = st.psd_decisions / sec
     launches = max(1, su.psd_launches)
     kern_ms = su.psd_kernel_ms / launches
@@ -250,36 +291,52 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": "v=45 Legendre-pair search (paf -2): gen at c=3, match, uncompress 3->1 of all "
-                               "matched pairs, equivalence filter; exhaustive, no input data",
+        "config": {"workload": "v=50 Golay-pair search (paf 0): gen at c=5, sort, match, uncompress 5->1 of all "
+                               "matched pairs, equivalence filter at length 50 with altnegative; exhaustive, "
+                               "no input data",
                    "order": ORDER, "compress": [C0, C1], "paf": PAF, "psd": "split-BF16 MFMA, FP32 accumulate, "
                    "FP64 band recheck and keys"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": profiled_traffic(),
-                     "kernel": "k_unc_filter", "kernel_ms_per_launch": kern_ms,
+                     "traffic_source": "profiles/%s (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
+                                       "not measured by this run)" % PROFILE,
+                     "kernel": "k_unc_filter", "kernel_ms_per_launch": kern_ms, "launches_per_step": launches,
                      "flops_per_candidate": FLOPS_PER_LEAF, "candidates_per_launch": su.psd_decisions / launches},
         "end_to_end_s": sec,
         "stage_ms_last_step": dict(STAGE_MS),
-        "search": {"pairs_c3": npairs3, "pairs_c1": npairs1, "unique": nunique, "psd_decisions": st.psd_decisions,
-                   "leaves": su.psd_decisions, "band_rechecks": st.band_rechecks, "tie_band": st.tie_band,
-                   "psd_pass": st.psd_pass, "pair_checks": st.pair_checks, "kernel_ms": st.kernel_ms},
+        "search": {"pairs_c%d" % C0: npairs0, "pairs_c%d" % C1: npairs1, "unique": nunique,
+                   "psd_decisions": st.psd_decisions, "leaves": su.psd_decisions, "band_rechecks": st.band_rechecks,
+                   "tie_band": st.tie_band, "psd_pass": st.psd_pass, "pair_checks": st.pair_checks,
+                   "kernel_ms": st.kernel_ms},
     }
+    pa = pb = None
+    if not (args.no_fp32 and args.no_cpu_baseline):
+        a, b, _ = ctx.generate(ORDER, C0, PAF)
+        rc, pa, pb, _ = ctx.match(ORDER, C0, PAF, ctx.sort_candidates(a), ctx.sort_candidates(b))
     if not args.no_fp32:
-        ctx.set_mode(g.GP_PSD_FP32)
-        sec32, (st32, su32, *_rest) = timed(ctx, g, 1, 0)
+        q0, q1 = FP32_SLICE
+        outs, res = {}, {}
+        for mode, name in ((g.GP_PSD_SPLIT_BF16, "split_bf16"), (g.GP_PSD_FP32, "fp32")):
+            ctx.set_mode(mode)
+            t0 = time.perf_counter()
+            oa, ob, _, _, s = ctx.uncompress(ORDER, C0, C1, PAF, pa[q0:q1], pb[q0:q1])
+            dt = time.perf_counter() - t0
+            outs[name] = "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob))
+            res[name] = {"leaves": s.psd_decisions, "psd_kernel_ms": s.psd_kernel_ms, "wall_ms": dt * 1e3,
+                         "candidates_per_s_kernel": s.psd_decisions / (s.psd_kernel_ms * 1e-3),
+                         "pairs_out": len(oa)}
         ctx.set_mode(g.GP_PSD_SPLIT_BF16)
-        line["fp32_path"] = {"value": st32.psd_decisions / sec32, "ms_per_step": sec32 * 1e3,
-                             "uncompress_kernel_ms": su32.psd_kernel_ms,
-                             "split_bf16_speedup_kernel": su32.psd_kernel_ms / max(su.psd_kernel_ms, 1e-9),
-                             "split_bf16_speedup_step": sec32 / sec}
+        line["fp32_path"] = {"slice": "uncompression 5->1 of the v=50 matched pairs [%d, %d) in file order" % (q0, q1),
+                             "identical_output": outs["split_bf16"] == outs["fp32"],
+                             "split_bf16": res["split_bf16"], "fp32": res["fp32"],
+                             "split_bf16_speedup_kernel": res["fp32"]["psd_kernel_ms"] /
+                             max(res["split_bf16"]["psd_kernel_ms"], 1e-9)}
     if not args.no_microbench:
         line["microbench"] = {"split_bf16": microbench(ctx, g, g.GP_PSD_SPLIT_BF16),
                               "fp32": microbench(ctx, g, g.GP_PSD_FP32)}
-    if not args.no_cpu_baseline:
-        a, b, _ = ctx.generate(ORDER, C0, PAF)
-        rc, pa, pb, _ = ctx.match(ORDER, C0, PAF, ctx.sort_candidates(a), ctx.sort_candidates(b))
-        line["cpu_baseline"] = cpu_baseline(pa, pb)
     ctx.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(pa, pb)
     print(json.dumps(line))
 
 

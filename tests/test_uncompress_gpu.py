@@ -222,9 +222,10 @@ def test_uncompress_v50_pair_slice_vs_oracle_fixture(gpu_ctx):
 
 
 @pytest.mark.parametrize("seq,c,c2,paf,njobs", [
+    # paf lifts the filter bound 2v - paf to 255 where the side's own bound passes nothing
     ((1, -1, 1, 5, 1, -1, -1), 5, 1, -2, 3),             # 2*10^5 leaves, split at depth 5 of 7
-    ((3, -1, 1, -3, 1, 1, -1, 1, -1, 1), 3, 1, -2, 2),   # split at depth 8 of 10
-    ((10, -2, 0, 4, -6, 2), 10, 5, 0, 4),                # 1800 leaves < 4000: every position set
+    ((3, -1, 1, -3, 1, 1, -1, 1, -1, 1), 3, 1, -195, 2),  # split at depth 8 of 10
+    ((-4, 6, -4, -2, 2, 0), 10, 5, -135, 4),             # 3600 leaves < 4000: every position set
     ((1, 1, -3, 3, -1), 5, 1, 0, 7),                     # 5000 leaves < 7000: every position set
 ])
 def test_uncompress_side_job_split_matches_oracle(gpu_ctx, seq, c, c2, paf, njobs):
