@@ -8,8 +8,6 @@
 #include "gp_internal.hpp"
 #include "gp_sort.hpp"
 #include "psd_core.hpp"
-#include "candkey.hpp"
-#include "linesort.hpp"
 #include "textfmt.hpp"
 
 using namespace gp;
@@ -94,53 +92,26 @@ int gp_generate(gp_ctx* ctx, int order, int compress, int paf, int proc_id, int 
 int gp_sort_candidates(gp_ctx* ctx, gp_cands* c) {
     if (!ctx) return GP_ERR_ARG;
     return guarded_call(ctx, [&] {
-        Phase ph("sort: import");
-        CandSet s = import_cands(c);
-        const int64_t m = s.size();
-        if (m == 0) return GP_OK;
-        const int n = s.n, K = s.nkey;
+        GP_HIP(hipSetDevice(ctx->device));
+        if (!c) throw ArgError("gp_sort_candidates: null argument");
+        const int64_t m = c->count;
+        if (m <= 0) return GP_OK;
+        const int n = c->n, K = c->nkey;
         if (K != n / 2) throw ArgError("gp_sort_candidates: nkey must be n/2");
-        // fixed-width keys in line order (no text): sorted, then equal keys
-        // (equal lines) dropped
-        ph.next("sort: keys");
-        const CandKeyer ck(n);
-        const int W = ck.W;
-        std::vector<uint64_t> key((size_t)m * W);
-        parallel_chunks(m, [&](int64_t i0, int64_t i1) {
-            for (int64_t i = i0; i < i1; ++i) ck.key(&s.words[i * n], &s.keys[i * K], &key[(size_t)i * W]);
-        });
-        ph.next("sort: sort");
-        auto kcmp = [&](int64_t x, int64_t y) {
-            const uint64_t* a = &key[(size_t)x * W];
-            const uint64_t* b = &key[(size_t)y * W];
-            for (int q = 0; q < W; ++q)
-                if (a[q] != b[q]) return a[q] < b[q] ? -1 : 1;
-            return 0;
-        };
-        std::vector<int64_t> idx(m);
-        std::iota(idx.begin(), idx.end(), 0);
-        parallel_sort(idx, [&](int64_t x, int64_t y) { return kcmp(x, y) < 0; });
-        ph.next("sort: uniq + copy");
-        // uniq: a line equal to its predecessor is dropped; kept rows are
-        // numbered by a prefix sum over chunks and copied on host threads
-        std::vector<uint8_t> keep(m);
-        parallel_chunks(m, [&](int64_t r0, int64_t r1) {
-            for (int64_t r = r0; r < r1; ++r) keep[r] = r == 0 || kcmp(idx[r], idx[r - 1]) != 0;
-        });
-        std::vector<int64_t> dst(m);
-        int64_t w = 0;
-        for (int64_t r = 0; r < m; ++r) {
-            dst[r] = w;
-            w += keep[r];
-        }
-        parallel_chunks(m, [&](int64_t r0, int64_t r1) {
-            for (int64_t r = r0; r < r1; ++r)
-                if (keep[r]) {
-                    std::memcpy(c->words + dst[r] * n, &s.words[idx[r] * n], n);
-                    std::memcpy(c->keys + dst[r] * K, &s.keys[idx[r] * K], K);
-                }
-        });
-        c->count = w;
+        if (!c->words || !c->keys) throw ArgError("gp_sort_candidates: null buffers");
+        // records in, sorted and unique records out, on the device
+        hipStream_t st = ctx->stream;
+        int8_t* dw = ctx->dev<int8_t>("s_w", (size_t)m * n);
+        uint8_t* dk = ctx->dev<uint8_t>("s_k", (size_t)m * K);
+        int8_t* ow = ctx->dev<int8_t>("s_ow", (size_t)m * n);
+        uint8_t* ok = ctx->dev<uint8_t>("s_ok", (size_t)m * K + 1);
+        GP_HIP(hipMemcpyAsync(dw, c->words, (size_t)m * n, hipMemcpyHostToDevice, st));
+        GP_HIP(hipMemcpyAsync(dk, c->keys, (size_t)m * K, hipMemcpyHostToDevice, st));
+        const int64_t kept = sort_uniq_cands(ctx, dw, dk, m, n, K, ow, ok);
+        GP_HIP(hipMemcpyAsync(c->words, ow, (size_t)kept * n, hipMemcpyDeviceToHost, st));
+        if (K > 0) GP_HIP(hipMemcpyAsync(c->keys, ok, (size_t)kept * K, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        c->count = kept;
         return GP_OK;
     });
 }

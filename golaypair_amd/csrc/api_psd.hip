@@ -147,6 +147,8 @@ void gp_destroy(gp_ctx* ctx) {
     for (auto& sl : ctx->slot_ev)
         for (hipEvent_t e : sl)
             if (e) (void)hipEventDestroy(e);
+    for (hipGraphExec_t g : ctx->unc_graph)
+        if (g) (void)hipGraphExecDestroy(g);
     hipStream_t s = ctx->stream, s2 = ctx->copy_stream;
     delete ctx;
     if (s) (void)hipStreamDestroy(s);

@@ -150,16 +150,6 @@ std::string pairs_text(const Pairs& p) {
     return s;
 }
 
-std::string sorted_unique_text(const std::string& text) {  // `sort | uniq` under LC_ALL=C
-    auto lines = split_lines(text);
-    std::sort(lines.begin(), lines.end());
-    lines.erase(std::unique(lines.begin(), lines.end()), lines.end());
-    std::string s;
-    s.reserve(text.size());
-    for (auto l : lines) { s.append(l); s.push_back('\n'); }
-    return s;
-}
-
 // Candidate lines "PSD<digits> v v v ... " -> records; the key digits are
 // recomputed on the GPU and checked against the line's key text.
 bool parse_cands(const std::string& text, int n, int bound, bool prefer_inverted, Cands& out, std::string& err) {
@@ -417,35 +407,26 @@ int main(int argc, char** argv) {
     if (s_sort || s_all) {  // main.cpp:69-79, sort.cpp:7-23
         std::cout << "Sorting Candidates\n";
         for (char ab : {'a', 'b'}) {
-            if (binary) {  // shards of either form -> one sorted, unique binary file
-                const int n = o.order / o.compress[0];
-                Cands all;
-                all.n = n;
-                for (int j = 0; j < o.job_count; ++j) {
-                    std::string t, err;
-                    Cands c;
-                    if (!read_file(cand(sort_in, ab, j), t) || !parse_any_cands(t, n, bound, ab == 'b', c, err)) {
-                        std::cerr << "Sort failed\n";
-                        return 1;
-                    }
-                    all.words.insert(all.words.end(), c.words.begin(), c.words.end());
-                    all.keys.insert(all.keys.end(), c.keys.begin(), c.keys.end());
-                }
-                gp_cands v = all.view();
-                if (all.size() > 0 && (rc = gp_sort_candidates(g_ctx, &v)) < 0) die_gp("gp_sort_candidates", rc);
-                all.words.resize(v.count * n);
-                all.keys.resize(v.count * (n / 2));
-                write_file(sorted_name(sort_out, ab), cands_binary(all));
-                continue;
-            }
-            std::string all_text;
+            // shards of either form -> records -> sorted, unique on the GPU in
+            // the byte order of the text lines -> one file in the chosen form
+            const int n = o.order / o.compress[0];
+            Cands all;
+            all.n = n;
             for (int j = 0; j < o.job_count; ++j) {
-                std::string t;
-                if (!read_file(cand(sort_in, ab, j), t)) { std::cerr << "Sort failed\n"; return 1; }
-                all_text += t;
-                if (!t.empty() && t.back() != '\n') all_text.push_back('\n');
+                std::string t, err;
+                Cands c;
+                if (!read_file(cand(sort_in, ab, j), t) || !parse_any_cands(t, n, bound, ab == 'b', c, err)) {
+                    std::cerr << "Sort failed\n";
+                    return 1;
+                }
+                all.words.insert(all.words.end(), c.words.begin(), c.words.end());
+                all.keys.insert(all.keys.end(), c.keys.begin(), c.keys.end());
             }
-            write_file(sorted_name(sort_out, ab), sorted_unique_text(all_text));
+            gp_cands v = all.view();
+            if (all.size() > 0 && (rc = gp_sort_candidates(g_ctx, &v)) < 0) die_gp("gp_sort_candidates", rc);
+            all.words.resize(v.count * n);
+            all.keys.resize(v.count * (n / 2));
+            write_file(sorted_name(sort_out, ab), binary ? cands_binary(all) : cands_text(all));
         }
     }
     if (s_match || s_all) {  // main.cpp:92-102, match_pairs.cpp:15-121
@@ -545,11 +526,16 @@ int main(int argc, char** argv) {
                     write_file(base + "a.sorted", cands_binary(ta));
                     write_file(base + "b.sorted", cands_binary(tb));
                 } else {
-                    const std::string xa = cands_text(ta), xb = cands_text(tb);
-                    write_file(base + "a", xa);
-                    write_file(base + "b", xb);
-                    write_file(base + "a.sorted", sorted_unique_text(xa));
-                    write_file(base + "b.sorted", sorted_unique_text(xb));
+                    write_file(base + "a", cands_text(ta));
+                    write_file(base + "b", cands_text(tb));
+                    for (Cands* c : {&ta, &tb}) {
+                        gp_cands v = c->view();
+                        if (c->size() > 0 && (rc = gp_sort_candidates(g_ctx, &v)) < 0) die_gp("gp_sort_candidates", rc);
+                        c->words.resize(v.count * N);
+                        c->keys.resize(v.count * (N / 2));
+                    }
+                    write_file(base + "a.sorted", cands_text(ta));
+                    write_file(base + "b.sorted", cands_text(tb));
                 }
             }
             report("uncompress", st);

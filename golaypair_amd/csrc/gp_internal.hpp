@@ -177,6 +177,8 @@ struct gp_ctx {
     hipStream_t copy_stream = nullptr;  // read-backs that must not wait for later work on `stream`
     hipEvent_t slot_ev[2][5] = {};      // per batch slot: filter start, filter end, band end, batch end, tables uploaded
     int64_t unc_ecap = 1 << 16;         // emitted-pair rows per in-flight batch (grows on overflow)
+    hipGraphExec_t unc_graph[2] = {};   // per batch slot: the band recheck + re-match chain
+    uint64_t unc_graph_launches = 0, unc_graph_instantiations = 0;
     std::vector<unsigned char> unc_side_buf;  // storage of the uncompress sides (reused across calls)
     int psd_mode = GP_PSD_SPLIT_BF16;
     std::string err;

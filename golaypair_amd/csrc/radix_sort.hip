@@ -96,18 +96,20 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint64_t* __res
     }
 }
 
-// keys/vals hold the input; the result ends in keys/vals (ping-pong through tmp).
-void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
-                      uint32_t* hist, int64_t n, int begin_bit, int end_bit) {
-    if (n <= 1) return;
+// keys/vals hold the input; the result ends in keys/vals (ping-pong through
+// tmp).  One 8-bit pass per entry of `shifts`, in the order given (least
+// significant digit first).
+void radix_sort_pairs_shifts(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
+                             uint32_t* hist, int64_t n, const int* shifts, int nshift) {
+    if (n <= 1 || nshift <= 0) return;
     if (n > 0xFFFFFFFFll) throw ArgError("radix sort: more than 2^32 elements");
     const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
     uint64_t* ki = keys;
     uint32_t* vi = vals;
     uint64_t* ko = tmp_keys;
     uint32_t* vo = tmp_vals;
-    int passes = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
+    for (int p = 0; p < nshift; ++p) {
+        const int shift = shifts[p];
         hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, st, ki, n, shift, hist, nblocks);
         hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, (int64_t)nblocks * 256);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, st, ki, vi, n, shift, hist, nblocks, ko,
@@ -116,10 +118,26 @@ void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* 
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    if (passes & 1) {
+    if (nshift & 1) {
         GP_HIP(hipMemcpyAsync(keys, ki, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
         GP_HIP(hipMemcpyAsync(vals, vi, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     }
+}
+
+void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
+                      uint32_t* hist, int64_t n, int begin_bit, int end_bit) {
+    int shifts[8];
+    int ns = 0;
+    for (int shift = begin_bit; shift < end_bit && ns < 8; shift += 8) shifts[ns++] = shift;
+    radix_sort_pairs_shifts(st, keys, vals, tmp_keys, tmp_vals, hist, n, shifts, ns);
+}
+
+// Exclusive scan of m uint32 values in place (one block); used by callers
+// outside this file for compaction offsets.
+void scan_exclusive_u32(hipStream_t st, uint32_t* v, int64_t m) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, v, m);
+    GP_HIP(hipGetLastError());
 }
 
 size_t radix_hist_bytes(int64_t n) { return (size_t)((n + kRsChunk - 1) / kRsChunk + 1) * 256 * 4; }

@@ -159,6 +159,15 @@ struct JoinCtl2 {
     unsigned long long ncand;   // candidate pairs (equal hashes)
 };
 
+// Clears two small device records (the join counters and the tie count of a
+// batch) with vector stores: a kernel node, so that the batch's captured graph
+// holds no memset node.
+__global__ void k_zero_ctl(unsigned int* __restrict__ a, int na, unsigned int* __restrict__ b, int nb) {
+    const int t = threadIdx.x;
+    if (t < na) a[t] = 0u;
+    if (t < nb) b[t] = 0u;
+}
+
 // Hash table size in use: the host's `tmask` + 1, or, with `jctl`, the
 // smallest power of two >= 2 * (B rows) (at least 64, at most tmask + 1).
 __device__ __forceinline__ uint64_t table_mask(const JoinCtl2* jctl, uint64_t tmask) {
@@ -1392,6 +1401,65 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
     return B;
 }
 
+// Stream capture of one batch's band recheck and re-match chain into the graph
+// of its buffer slot.  The first batch of a slot instantiates the graph; later
+// batches update it in place (hipGraphExecUpdate), or instantiate it again if
+// the update is refused.  GOLAY_UNC_GRAPH=0 launches the kernels directly.
+// A capture that ends by an exception is closed and dropped.
+struct BatchGraph {
+    gp_ctx* ctx;
+    int slot;
+    bool on;
+    bool capturing = false;
+    hipStream_t st = nullptr;
+    BatchGraph(gp_ctx* c, int s) : ctx(c), slot(s) {
+        const char* e = std::getenv("GOLAY_UNC_GRAPH");
+        on = !(e && e[0] == '0');
+    }
+    void begin(hipStream_t s) {
+        if (!on) return;
+        st = s;
+        GP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+    }
+    void end_and_launch(hipStream_t s) {
+        hipGraph_t g = nullptr;
+        capturing = false;
+        GP_HIP(hipStreamEndCapture(s, &g));
+        hipGraphExec_t& ex = ctx->unc_graph[slot];
+        bool ok = false;
+        if (ex) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res;
+            ok = hipGraphExecUpdate(ex, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!ok) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(ex);
+                ex = nullptr;
+            }
+        }
+        if (!ok) {
+            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                ex = nullptr;
+                GP_HIP(e);
+            }
+            ++ctx->unc_graph_instantiations;
+        }
+        (void)hipGraphDestroy(g);
+        GP_HIP(hipGraphLaunch(ex, s));
+        ++ctx->unc_graph_launches;
+    }
+    ~BatchGraph() {
+        if (capturing) {  // an exception left the capture open
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+        }
+    }
+};
+
 // A batch whose filter, band recheck and join are queued on the context's
 // stream with device-side counts, in the buffers of one of two slots.
 struct UncInflight {
@@ -1430,11 +1498,6 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     GP_HIP(hipEventRecord(ev[0], st));
     launch_filters(ctx, U, tb, F->ctl, d_surv, F->scap, d_band, F->bcap);
     GP_HIP(hipEventRecord(ev[1], st));
-    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
-                       ctx->tables.d_trig64.as<double>(), d_toff, d_band, F->ctl, d_surv, F->scap, (int64_t)0,
-                       &F->ctl->band, F->bcap);
-    GP_HIP(hipGetLastError());
-    GP_HIP(hipEventRecord(ev[2], st));
     // join over at most rcap survivors (the batch is joined again synchronously
     // when more survive); the part of the table in use is sized on the device
     // from the B rows counted by k_unc_rows and stays at most half full
@@ -1457,8 +1520,22 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     F->k = ctx->dev<uint8_t>("u_ek" + sfx, (size_t)2 * F->ecap * B.K);
     F->grp = ctx->dev<int32_t>("u_egrp" + sfx, (size_t)F->ecap);
     F->ties = ctx->dev<unsigned int>("u_eties" + sfx, 1);
-    GP_HIP(hipMemsetAsync(F->jctl, 0, sizeof(JoinCtl2), st));
-    GP_HIP(hipMemsetAsync(F->ties, 0, 4, st));
+    // The band recheck and the re-match chain have the same nodes in every
+    // batch: they are captured into a graph, which updates the slot's
+    // instantiated graph (kernel arguments and grids change per batch) and is
+    // launched as one unit.  Everything they use is allocated above.
+    BatchGraph graph(ctx, slot);
+    if (graph.on) GP_HIP(hipEventRecord(ev[2], st));  // the band time is counted with the join's
+    graph.begin(st);
+    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
+                       ctx->tables.d_trig64.as<double>(), d_toff, d_band, F->ctl, d_surv, F->scap, (int64_t)0,
+                       &F->ctl->band, F->bcap);
+    GP_HIP(hipGetLastError());
+    if (!graph.on) GP_HIP(hipEventRecord(ev[2], st));
+    static_assert(sizeof(JoinCtl2) % 4 == 0);
+    hipLaunchKernelGGL(k_zero_ctl, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned int*>(F->jctl),
+                       (int)(sizeof(JoinCtl2) / 4), F->ties, 1);
+    GP_HIP(hipGetLastError());
     const unsigned long long* nsurv = &F->ctl->surv;
     const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, d_surv,
@@ -1479,6 +1556,7 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
                        oa, ob, &F->jctl->out, F->ecap, ctx->tables.d_trig64.as<double>() + B.trig_off, B.bound, F->w,
                        F->k, F->grp, F->ties);
     GP_HIP(hipGetLastError());
+    if (graph.on) graph.end_and_launch(st);
     GP_HIP(hipEventRecord(ev[3], st));
     return F;
 }

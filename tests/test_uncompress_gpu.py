@@ -247,3 +247,24 @@ def test_uncompress_side_job_split_matches_oracle(gpu_ctx, seq, c, c2, paf, njob
     assert len(whole) > 0
     cands, _, _ = gpu_ctx.uncompress_side(seq, c, c2, paf, 1, njobs, njobs)   # no job of that id
     assert len(cands) == 0
+
+
+@pytest.mark.parametrize("batch_leaves", [64, 1 << 14])
+def test_uncompress_batch_graph_matches_eager(gpu_ctx, monkeypatch, batch_leaves):
+    """Each batch's band recheck and re-match chain runs as a captured hipGraph that
+    updates its slot's instantiated graph; GOLAY_UNC_GRAPH=0 launches the same
+    kernels directly.  Small batches give many graph updates and launches, with the
+    batch's eager work (uploads, filter launches, read-backs) and synchronizations
+    between them: every run must give the oracle's pairs and temporary files."""
+    order, compress, paf = 26, [2, 1], 0
+    wd = _oracle_dir(order, compress, paf, ["gen", "sort", "match", "uncompress"])
+    c, c2 = compress
+    pa, pb = _read_pairs(os.path.join(wd, "%d-pairs-%d" % (order, c)), order // c)
+    want = open(os.path.join(wd, "%d-pairs-%d" % (order, c2))).read()
+    monkeypatch.setenv("GOLAY_UNC_BATCH_LEAVES", str(batch_leaves))
+    for mode in ("1", "0", "1", "1"):
+        monkeypatch.setenv("GOLAY_UNC_GRAPH", mode)
+        oa, ob, ta, tb, st = gpu_ctx.uncompress(order, c, c2, paf, pa, pb, want_tmp=True)
+        assert "".join(g.pair_line(x, y) + "\n" for x, y in zip(oa, ob)) == want, mode
+        assert "".join(l + "\n" for l in ta.lines()) == open(os.path.join(wd, "uncompress_tmp_a")).read()
+        assert st.tie_band == 0
