@@ -83,6 +83,9 @@ SIGNATURES = {
     "gp_psd_filter_batch": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int8), ctypes.c_int, _P(ctypes.c_int32),
                                            _P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_int64,
                                            _P(ctypes.c_uint8), _P(ctypes.c_float), _P(GpStats)]),
+    "gp_psd_filter_batch_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p,
+                                                  _P(GpStats)]),
     "gp_psd_microbench": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, _P(ctypes.c_int32),
                                          ctypes.c_int, _P(ctypes.c_uint64), _P(ctypes.c_uint64),
                                          _P(ctypes.c_uint8), ctypes.c_int64, _P(GpStats)]),

@@ -119,6 +119,7 @@ int gp_create(gp_ctx** out, int device) {
         hipDeviceProp_t prop;
         GP_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
+        psd_static_setup(device);  // device address of the static scratch of gp_psd_filter_batch_static
         GP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         GP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
         for (auto& sl : ctx->slot_ev)

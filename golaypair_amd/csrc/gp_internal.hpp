@@ -208,6 +208,10 @@ struct gp_ctx {
 
 namespace gp {
 
+// Looks up (once per device, with that device current) the device address of
+// the static scratch array used by gp_psd_filter_batch_static.
+void psd_static_setup(int device);
+
 // Error bound on Re/Im of the split-BF16 contraction for words with
 // sum |x| <= s1 using table `d`.
 double split_bf16_E(const PsdTableDesc& d, double s1);

@@ -170,9 +170,9 @@ __device__ __forceinline__ void psd_tile(const int8_t* __restrict__ words, int s
         const int64_t row = td.first_row + col;
         decision[row] = d;
         if (maxpsd) maxpsd[row] = m;
-        if (d == kBand) {
+        if (d == kBand) {  // without a band list the rows keep their band code in `decision`
             unsigned int i = atomicAdd(&ctl->count, 1u);
-            band_rows[i] = row;
+            if (band_rows) band_rows[i] = row;
         }
     }
 }
@@ -366,3 +366,204 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
 }
 
 }  // namespace gp
+
+// ---------------------------------------------------------------------------
+// The batched PSD filter without allocation (gp_psd_filter_batch_static).
+//
+// Everything the call needs beyond the caller's words and pass flags lives in
+// `scratch`, one file-scope device array per GPU, held until the process ends:
+//   [0, 256)                  NaHead: band / tie / pass counters, the row-sum
+//                             bound S1, the thresholds and the table descriptor
+//   [256, 256 + 16 KiB)       the split-BF16 A fragments of the word length
+//   [.., + 1 KiB)             its FP64 twiddles.
+// Host code refers to it only through the address hipGetSymbolAddress gives
+// (kept per device by psd_static_setup); the kernels take that address.  Every
+// call on one GPU shares the array: calls that overlap on one GPU race.
+
+namespace gp {
+
+struct NaHead {
+    BandCtl ctl;                  // band rows counted by k_psd_mfma, FP64 ties
+    unsigned long long passed;    // rows accepted
+    unsigned int s1;              // max over the rows of sum |x_p|
+    unsigned int pad;
+    FilterThr thr;
+    PsdTableDesc desc;            // frag_off = trig_off = 0: the regions below
+};
+constexpr size_t kNaHeadBytes = 256;
+constexpr size_t kNaFragCount = 2 * 8 * 64;  // M-tiles (<= 2) x 2 nb (<= 8) x 64 lanes, n <= 64
+constexpr size_t kNaFragOff = kNaHeadBytes;
+constexpr size_t kNaTrigOff = kNaFragOff + kNaFragCount * sizeof(uint4);
+constexpr size_t kNaBytes = kNaTrigOff + 2 * kMaxN * sizeof(double);
+static_assert(sizeof(NaHead) <= kNaHeadBytes, "scratch header");
+constexpr int kNaMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kNaMaxDevices] = {};  // device address of `scratch`, per device
+int g_na_n[kNaMaxDevices] = {};               // word length whose table `scratch` holds
+}  // namespace
+
+void psd_static_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (device < 0 || device >= kNaMaxDevices) throw ArgError("device ordinal beyond the static scratch table");
+    if (g_na_ptr[device]) return;
+    void* p = nullptr;
+    GP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[device] = static_cast<unsigned char*>(p);
+}
+
+__global__ void k_na_reset(NaHead* h) {
+    if (threadIdx.x == 0) {
+        h->ctl.count = 0;
+        h->ctl.ties = 0;
+        h->passed = 0;
+        h->s1 = 0;
+    }
+}
+
+// S1 = max over the rows of sum |x_p| (the row-sum bound of the error analysis).
+__global__ void __launch_bounds__(256) k_na_s1(const int8_t* __restrict__ words, int stride, int n, int64_t count,
+                                               NaHead* h) {
+    unsigned int best = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < count; r += (int64_t)gridDim.x * blockDim.x) {
+        const int8_t* w = words + r * stride;
+        unsigned int a = 0;
+        for (int p = 0; p < n; ++p) a += (unsigned int)abs((int)w[p]);
+        best = max(best, a);
+    }
+    for (int o = 32; o; o >>= 1) best = max(best, (unsigned int)__shfl_xor((int)best, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&h->s1, best);
+}
+
+// Thresholds from S1 as the host path makes them (split_bf16_E, make_thr).
+__global__ void k_na_thr(NaHead* h, double thr) {
+    if (threadIdx.x != 0) return;
+    const double adds = 32.0 * h->desc.nb + 1.0;
+    const double E = (double)h->s1 * ((double)h->desc.rho_max + 1.004 * adds * 0x1p-23);
+    const double eps = psd_eps(E, thr);
+    h->thr.thr = thr;
+    h->thr.hi = nextafterf((float)(thr + eps), INFINITY);
+    h->thr.lo = nextafterf((float)(thr - eps), -INFINITY);
+}
+
+// Rows left in the band by k_psd_mfma are settled in FP64 in place; accepted
+// rows are counted.
+__global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ words, int stride, int64_t count,
+                                                 const double* __restrict__ trig64, NaHead* h,
+                                                 uint8_t* __restrict__ pass) {
+    const int n = h->desc.n;
+    const double t = h->thr.thr;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < count; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = base + threadIdx.x;
+        unsigned long long acc = 0;
+        if (r < count) {
+            uint8_t d = pass[r];
+            if (d == kBand) {
+                const double m = fp64_max_psd(words + r * stride, n, trig64, trig64 + n);
+                if (fabs(m - t) < 1e-9) atomicAdd(&h->ctl.ties, 1u);
+                d = m > t ? kReject : kAccept;
+                pass[r] = d;
+            }
+            acc = d == kAccept;
+        }
+        wave_atomic_add(&h->passed, acc);
+    }
+}
+
+}  // namespace gp
+
+extern "C" int gp_psd_filter_batch_static(gp_ctx* ctx, const int8_t* d_words, int stride, int n, int order, int paf,
+                                          int64_t count, uint8_t* d_pass, gp_stats* stats) {
+    using namespace gp;
+    if (!ctx) return GP_ERR_ARG;
+    try {
+        GP_HIP(hipSetDevice(ctx->device));
+        if (n < 2 || n > kMaxN)
+            throw ArgError("gp_psd_filter_batch_static: word length " + std::to_string(n) +
+                           " outside [2, 64] (the static scratch holds one table of at most 64 entries)");
+        const int nb = (n + 15) / 16;
+        if (count < 0 || (count > 0 && (!d_words || !d_pass)))
+            throw ArgError("gp_psd_filter_batch_static: bad buffers");
+        if (stride < 16 * nb || stride % 8 || reinterpret_cast<uintptr_t>(d_words) % 8)
+            throw ArgError("gp_psd_filter_batch_static: rows must be 8-byte aligned with stride >= 16*ceil(n/16)");
+        unsigned char* base = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_na_mu);
+            base = g_na_ptr[ctx->device];
+        }
+        if (!base) throw ArgError("gp_psd_filter_batch_static: static scratch not set up on this device");
+        if (count == 0) return GP_OK;
+        hipStream_t st = ctx->stream;
+        NaHead* h = reinterpret_cast<NaHead*>(base);
+        const uint4* frags = reinterpret_cast<const uint4*>(base + kNaFragOff);
+        const double* trig64 = reinterpret_cast<const double*>(base + kNaTrigOff);
+        {
+            // the table of this word length, copied in when the scratch holds another
+            std::lock_guard<std::mutex> lk(g_na_mu);
+            if (g_na_n[ctx->device] != n) {
+                const int id = ctx->tables.get(n);
+                PsdTableDesc d = ctx->tables.desc[id];
+                const size_t nfrag = (size_t)d.mt * 2 * d.nb * 64;
+                if (nfrag > kNaFragCount) throw ArgError("gp_psd_filter_batch_static: table beyond the scratch");
+                GP_HIP(hipMemcpyAsync(base + kNaFragOff, ctx->tables.frags.data() + d.frag_off, nfrag * sizeof(uint4),
+                                      hipMemcpyHostToDevice, st));
+                GP_HIP(hipMemcpyAsync(base + kNaTrigOff, ctx->tables.trig64.data() + d.trig_off,
+                                      2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+                d.frag_off = 0;
+                d.trig_off = 0;
+                GP_HIP(hipMemcpyAsync(&h->desc, &d, sizeof d, hipMemcpyHostToDevice, st));
+                GP_HIP(hipStreamSynchronize(st));
+                g_na_n[ctx->device] = n;
+            }
+        }
+        const int blocks = (int)std::min<int64_t>((count + 255) / 256, (int64_t)ctx->num_cus * 8);
+        hipLaunchKernelGGL(k_na_reset, dim3(1), dim3(64), 0, st, h);
+        hipLaunchKernelGGL(k_na_s1, dim3(blocks), dim3(256), 0, st, d_words, stride, n, count, h);
+        hipLaunchKernelGGL(k_na_thr, dim3(1), dim3(64), 0, st, h, 2.0 * order - paf + 0.001);
+        GP_HIP(hipGetLastError());
+        SegTiles T{};
+        T.nseg = 1;
+        T.table[0] = 0;
+        T.first_row[0] = 0;
+        T.rows[0] = count;
+        T.tile0[0] = 0;
+        const int64_t ntiles = (count + 31) / 32;
+        T.tile0[1] = ntiles;
+        const int64_t waves = std::min<int64_t>(ntiles, (int64_t)ctx->num_cus * 16);
+        GP_HIP(hipEventRecord(ctx->ev0, st));
+        hipLaunchKernelGGL(k_psd_mfma, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0, st,
+                           d_words, stride, T, ntiles, &h->desc, frags, &h->thr, d_pass, &h->ctl,
+                           static_cast<int64_t*>(nullptr), static_cast<float*>(nullptr));
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipEventRecord(ctx->ev1, st));
+        hipLaunchKernelGGL(k_na_band, dim3(blocks), dim3(256), 0, st, d_words, stride, count, trig64, h, d_pass);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipEventRecord(ctx->ev2, st));
+        NaHead hh;
+        GP_HIP(hipMemcpyAsync(&hh, h, sizeof hh, hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        if (stats) {
+            float ms_main = 0, ms_band = 0;
+            GP_HIP(hipEventElapsedTime(&ms_main, ctx->ev0, ctx->ev1));
+            GP_HIP(hipEventElapsedTime(&ms_band, ctx->ev1, ctx->ev2));
+            stats->psd_decisions += (uint64_t)count;
+            stats->psd_pass += hh.passed;
+            stats->band_rechecks += hh.ctl.count;
+            stats->tie_band += hh.ctl.ties;
+            stats->psd_kernel_ms += ms_main;
+            stats->band_kernel_ms += ms_band;
+            stats->psd_launches += 1;
+            stats->kernel_ms += ms_main + ms_band;
+        }
+        return GP_OK;
+    } catch (const ArgError& e) {
+        ctx->err = e.what();
+        return GP_ERR_ARG;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return GP_ERR_HIP;
+    }
+}

@@ -161,6 +161,24 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
                         const int32_t* orders, const int32_t* pafs, int64_t count,
                         uint8_t* pass, float* max_psd, gp_stats* stats);
 
+/* The batched PSD filter of one word length without any allocation: words and
+ * pass flags are DEVICE memory (d_words: count rows of n int8 entries at
+ * `stride` bytes, 8-byte aligned, stride >= 16*ceil(n/16); d_pass: count
+ * bytes), pass[i] = 1 iff PSD_i(k) <= 2*order - paf + 0.001 for every
+ * 1 <= k <= n/2, computed on the split-BF16 path with the FP64 recheck of the
+ * error band (the same decisions as gp_psd_filter_batch).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that a
+ * caller captures into a graph, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call uses no hipMalloc / hipFree; its working memory is a
+ * static device array of about 18 KiB per GPU, held until the process ends
+ * (word length at most 64, the size of the one PSD table it holds).  All calls on
+ * one GPU share that array: calls that overlap on one GPU race.  The first call
+ * for a word length copies that length's table into the array (a later call
+ * with the same length does not).  The call returns when its work on the
+ * context's stream is done; stats (may be NULL) receives the decision counts. */
+int gp_psd_filter_batch_static(gp_ctx* ctx, const int8_t* d_words, int stride, int n, int order, int paf,
+                               int64_t count, uint8_t* d_pass, gp_stats* stats);
+
 /* FP64 key digits of caller words, the numbers printed after "PSD" by
  * write_seq_psd / write_seq_psd_invert (src/utils/io.cpp:14-36, called at
  * generate_hybrid.cpp:144,152 and uncompression.cpp:200-204):
