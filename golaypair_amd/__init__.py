@@ -68,6 +68,8 @@ SIGNATURES = {
     "gp_destroy": (None, [ctypes.c_void_p]),
     "gp_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "gp_set_psd_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gp_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    "gp_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_int64)]),
     "gp_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, _P(GpCands), _P(GpCands), _P(GpStats)]),
     "gp_sort_candidates": (ctypes.c_int, [ctypes.c_void_p, _P(GpCands)]),

@@ -120,6 +120,7 @@ int gp_create(gp_ctx** out, int device) {
         GP_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
         psd_static_setup(device);  // device address of the static scratch of gp_psd_filter_batch_static
+        load_tuning(ctx);
         GP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         GP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
         for (auto& sl : ctx->slot_ev)

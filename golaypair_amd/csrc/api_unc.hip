@@ -112,7 +112,8 @@ int gp_uncompress(gp_ctx* ctx, int order, int compress, int new_compress, int pa
         SideHost* const all = reinterpret_cast<SideHost*>(ctx->unc_side_buf.data());
         std::vector<int> group((size_t)(2 * P));
         for (size_t i = 0; i < group.size(); ++i) group[i] = (int)(i / 2);
-        uint64_t leaf_budget = kLeafBudget;
+        uint64_t leaf_budget = (uint64_t)std::clamp<int64_t>(
+            tuned(ctx, "unc_leaf_budget/" + std::to_string(N), (int64_t)kLeafBudget), 1 << 16, (int64_t)1 << 34);
         if (const char* e = std::getenv("GOLAY_UNC_BATCH_LEAVES"))  // smaller batches (tests)
             leaf_budget = std::max<uint64_t>(4, std::strtoull(e, nullptr, 10));
         UncTrig trig;

@@ -418,7 +418,9 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     // ranks per lane: about 8 wave tasks per resident wave (so that the last
     // round of tasks is short), at least 16 ranks to amortise each lane's unrank
     const int64_t target_tasks = (int64_t)ctx->num_cus * 32 * 8;
-    const int per_lane = (int)std::clamp<int64_t>((total + 64 * target_tasks - 1) / (64 * target_tasks), 16, 128);
+    const int heuristic = (int)std::clamp<int64_t>((total + 64 * target_tasks - 1) / (64 * target_tasks), 16, 128);
+    const int per_lane = (int)std::clamp<int64_t>(
+        tuned(ctx, "gen_ranks_per_lane/" + std::to_string(n) + "/" + std::to_string(k), heuristic), 4, 4096);
     for (size_t id = 0; id < segs.size(); ++id)
         for (int64_t r = 0; r < segs[id].count; r += 64 * per_lane)
             tasks.push_back(GenTask{(int32_t)id, per_lane, r, segs[id].count});

@@ -178,6 +178,7 @@ struct gp_ctx {
     hipEvent_t slot_ev[2][5] = {};      // per batch slot: filter start, filter end, band end, batch end, tables uploaded
     int64_t unc_ecap = 1 << 16;         // emitted-pair rows per in-flight batch (grows on overflow)
     hipGraphExec_t unc_graph[2] = {};   // per batch slot: the band recheck + re-match chain
+    std::map<std::string, int64_t> tuning;  // measured launch / batch parameters (tuning.hip)
     uint64_t unc_graph_launches = 0, unc_graph_instantiations = 0;
     std::vector<unsigned char> unc_side_buf;  // storage of the uncompress sides (reused across calls)
     int psd_mode = GP_PSD_SPLIT_BF16;
@@ -207,6 +208,11 @@ struct gp_ctx {
 };
 
 namespace gp {
+
+// Tuning table (tuning.hip): loaded from the tuning file when a context is
+// created; tuned(ctx, key, dflt) is the table's value or dflt.
+void load_tuning(gp_ctx* ctx);
+int64_t tuned(const gp_ctx* ctx, const std::string& key, int64_t dflt);
 
 // Looks up (once per device, with that device current) the device address of
 // the static scratch array used by gp_psd_filter_batch_static.

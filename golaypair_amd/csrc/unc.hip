@@ -1321,7 +1321,8 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
     if (!U.fp32) {
         for (const UncClass& c : U.classes) {
             // as many waves as the class keeps resident (its occupancy hint, 4 SIMDs per CU)
-            const int per_simd = unc_waves_per_simd(c.nbs, c.mt, c.half);
+            const int cap = unc_waves_per_simd(c.nbs, c.mt, c.half);
+            const int per_simd = (int)std::clamp<int64_t>(tuned(ctx, "unc_waves/" + std::to_string(U.N), cap), 1, cap);
             const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 4 * per_simd);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
             launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);

@@ -86,6 +86,18 @@ const char* gp_last_error(const gp_ctx* ctx);
 /* Select GP_PSD_SPLIT_BF16 (default) or GP_PSD_FP32 for later calls. */
 int gp_set_psd_mode(gp_ctx* ctx, int mode);
 
+/* Tuning table of the context (no counterpart in the reference): measured launch
+ * and batch parameters by key, e.g. "unc_waves/50" (filter waves per SIMD at
+ * finer length 50), "unc_leaf_budget/50" (leaves per uncompression batch),
+ * "gen_ranks_per_lane/10/6" (gen ranks per lane at length 10, 6 letters).  A
+ * context starts with the table of the tuning file (GOLAY_TUNING, else
+ * golaypair_amd/tuning.tsv, written by python -m golaypair_amd.autotune); every
+ * setting gives the same results, only the time changes.  gp_set_tuning with a
+ * negative value removes the key (the built-in default applies);
+ * gp_get_tuning returns GP_NO_INPUT for a key the table does not hold. */
+int gp_set_tuning(gp_ctx* ctx, const char* key, int64_t value);
+int gp_get_tuning(const gp_ctx* ctx, const char* key, int64_t* value);
+
 /* Candidate generation for one compression level.
  * Replaces int generate_hybrid(int ORDER, int COMPRESS, int PAF_CONSTANT,
  *   std::ofstream& out_a, std::ofstream& out_b, int PROC_ID, int PROC_NUM)
