@@ -266,6 +266,225 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
     wave_atomic_add(&ctl->words, nwords);
 }
 
+// Fused gen (split-BF16 path): the enumeration and group tests of k_gen_enum,
+// then, with no word or row leaving the chip, the PSD filter of the canonical
+// words 32 at a time, the FP64 settling of the words inside the error band and
+// the FP64 keys of the accepted ones, appended as (word, key, side) rows.
+//   - canonical words (with their sides: bit 0 A, bit 1 B) wait in a second
+//     per-wave LDS queue; 32 of them make one MFMA tile, whose B operand is
+//     unpacked in registers from the 4-bit letter indices (x = a0 + 2 d);
+//   - the length-n table's A fragments stay in registers, its FP64 twiddles in
+//     LDS; one word per lane of the lower half-wave settles the band
+//     (fp64_max_psd) and computes the keys (fp64_psd_k, rint) of an accepted
+//     word, one row per side it was kept for (A: rint PSD_k, B: bound - rint).
+// The rows come out in no particular order (the host puts them in file order).
+struct GenFuse {
+    FilterThr thr;   // thresholds of the length-n table (S1 = n c)
+    int32_t a0;      // letter of index 0 (-c)
+    int32_t bound;   // 2 order - paf (inverted B keys)
+    int64_t cap;     // output rows
+};
+struct GenFuseCtl {
+    unsigned long long words;  // words visited
+    unsigned long long canon;  // canonical (word, side) rows, the PSD decisions
+    unsigned long long rows;   // accepted rows (slots used; may exceed cap)
+    unsigned long long band;   // words settled in FP64
+    unsigned int ties;         // FP64 values within 1e-9 of the threshold or of a rounding half-point
+    unsigned int pad;
+};
+
+template <class Wd, int NB>
+__global__ void __launch_bounds__(256) k_gen_fused(GenParams P, GenFuse F, const GenSeg* __restrict__ segs,
+                                                   const GenTask* __restrict__ tasks, int64_t ntasks,
+                                                   const uint64_t* __restrict__ cnt, int cstride,
+                                                   const uint32_t* __restrict__ shard_bits,
+                                                   const uint4* __restrict__ afrag, const double* __restrict__ trig64,
+                                                   int8_t* __restrict__ ow, uint8_t* __restrict__ okeys,
+                                                   uint8_t* __restrict__ oside, GenFuseCtl* ctl) {
+    __shared__ Wd queue_all[4][128];      // necklaces waiting for the group tests
+    __shared__ Wd canon_all[4][128];      // canonical words waiting for the PSD tile
+    __shared__ uint8_t cside_all[4][128];
+    __shared__ int8_t wl_all[4][32][32];  // letters of the tile's words (FP64 work)
+    __shared__ double tl[2 * 32];         // FP64 twiddles: cos[n], sin[n]
+    const int n = P.n, k = P.k, K = n >> 1;
+    for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) tl[t] = trig64[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Wd* const queue = queue_all[wv];
+    Wd* const canon = canon_all[wv];
+    uint8_t* const cside = cside_all[wv];
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const Wd mask = mask_n<Wd>(n);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long nwords = 0, ncanon = 0, nband = 0;
+    unsigned int nties = 0;
+    uint4 af[1][2 * NB];
+#pragma unroll
+    for (int s = 0; s < 2 * NB; ++s) af[0][s] = afrag[s * 64 + lane];
+    int cq = 0;  // canonical words queued (wave-uniform)
+    // PSD tile of canon[0 .. m) (m <= 32), decisions, FP64 work, row append
+    auto psd_tile = [&](int m) {
+        const int col = lane & 31, half = lane >> 5;
+        const bool valid = col < m;
+        const Wd w = valid ? canon[col] : (Wd)0;
+        uint4 bop[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            uint32_t h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p0 = 16 * b + 8 * half + 2 * i;
+                const int x0 = valid && p0 < n ? F.a0 + 2 * digit_at(w, n, p0) : 0;
+                const int x1 = valid && p0 + 1 < n ? F.a0 + 2 * digit_at(w, n, p0 + 1) : 0;
+                h[i] = bf16_bits(x0) | (bf16_bits(x1) << 16);
+            }
+            bop[b] = make_uint4(h[0], h[1], h[2], h[3]);
+        }
+        f32x16 acc[1];
+        acc[0] = f32x16{};
+        mfma_tile_reg<NB, 1>(af, bop, acc);
+        const float mx = tile_max_psd<1>(acc);  // both halves: max_k PSD_k of word col
+        bool acc_ok = false;
+        if (half == 0 && valid && !(mx > F.thr.hi)) {
+            int8_t* wl = wl_all[wv][col];
+            for (int p = 0; p < n; ++p) wl[p] = (int8_t)(F.a0 + 2 * digit_at(w, n, p));
+            if (mx < F.thr.lo) {
+                acc_ok = true;
+            } else {  // inside the error band: FP64 (the order of k_psd_fp64)
+                ++nband;
+                const double m64 = fp64_max_psd(wl, n, tl, tl + n);
+                if (fabs(m64 - F.thr.thr) < 1e-9) ++nties;
+                acc_ok = !(m64 > F.thr.thr);
+            }
+        }
+        const int sd = valid ? cside[col] : 0;
+        const bool rowA = acc_ok && (sd & 1), rowB = acc_ok && (sd & 2);
+        const unsigned long long ba = __ballot(rowA), bb = __ballot(rowB);
+        const int tot = __popcll(ba) + __popcll(bb);
+        if (tot) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&ctl->rows, (unsigned long long)tot);
+            base = __shfl(base, 0);
+            if (acc_ok && (rowA || rowB)) {
+                const int8_t* wl = wl_all[wv][col];
+                const unsigned long long sa = base + __popcll(ba & below);
+                const unsigned long long sb = base + __popcll(ba) + __popcll(bb & below);
+                for (int j = 1; j <= K; ++j) {
+                    const double pk = fp64_psd_k(wl, n, tl, tl + n, j);
+                    const double f = pk - floor(pk);
+                    if (fabs(f - 0.5) < 1e-9) ++nties;
+                    const int v = (int)rint(pk);
+                    if (rowA && (int64_t)sa < F.cap) okeys[sa * K + j - 1] = (uint8_t)v;
+                    if (rowB && (int64_t)sb < F.cap) okeys[sb * K + j - 1] = (uint8_t)(F.bound - v);
+                }
+                for (int p = 0; p < n; ++p) {
+                    if (rowA && (int64_t)sa < F.cap) ow[sa * n + p] = wl[p];
+                    if (rowB && (int64_t)sb < F.cap) ow[sb * n + p] = wl[p];
+                }
+                if (rowA && (int64_t)sa < F.cap) oside[sa] = 0;
+                if (rowB && (int64_t)sb < F.cap) oside[sb] = 1;
+            }
+        }
+    };
+    // group tests of queue[0 .. qc) (qc <= 64, one per lane); kept words join the canonical queue
+    auto drain = [&](int qc, int side) {
+        const bool have = lane < qc;
+        const Wd w = have ? queue[lane] : (Wd)0;
+        bool keepA = false, keepB = false;
+        if (have) {
+            bool dih = true;
+            for (int j = 0; j < P.nunits_b && dih; ++j)
+                dih = le_all_rotations(w, decimate(w, n, P.units_b[j]), n, mask, 0);
+            keepB = dih && (side & 2);
+            if (dih && (side & 1)) {
+                bool agl = true;
+                for (int j = 0; j < P.nunits_a && agl; ++j) {
+                    const int u = P.units_a[j];
+                    if (u == n - 1) continue;  // covered by the dihedral test
+                    agl = le_all_rotations(w, decimate(w, n, u), n, mask, 0);
+                }
+                keepA = agl;
+            }
+        }
+        const bool keep = keepA || keepB;
+        ncanon += (unsigned long long)keepA + (unsigned long long)keepB;
+        const unsigned long long bk = __ballot(keep);
+        if (keep) {
+            const int at = cq + __popcll(bk & below);
+            canon[at] = w;
+            cside[at] = (uint8_t)((keepA ? 1 : 0) | (keepB ? 2 : 0));
+        }
+        cq += __popcll(bk);
+        while (cq >= 32) {
+            psd_tile(32);
+            // the rest (fewer than 96) moves to the front
+            const int rest = cq - 32;
+            Wd t0 = 0, t1 = 0;
+            uint8_t s0 = 0, s1 = 0;
+            if (lane < rest) { t0 = canon[32 + lane]; s0 = cside[32 + lane]; }
+            if (lane + 64 < rest) { t1 = canon[96 + lane]; s1 = cside[96 + lane]; }
+            if (lane < rest) { canon[lane] = t0; cside[lane] = s0; }
+            if (lane + 64 < rest) { canon[64 + lane] = t1; cside[64 + lane] = s1; }
+            cq = rest;
+        }
+    };
+    for (int64_t ti = wave; ti < ntasks; ti += nwaves) {
+        const GenTask T = tasks[ti];
+        const GenSeg S = segs[T.seg];
+        const int side = __builtin_amdgcn_readfirstlane(S.side);
+        int64_t r0 = T.start + (int64_t)lane * T.per_lane;
+        int64_t r1 = min(r0 + (int64_t)T.per_lane, T.end);
+        const Wd head = (Wd)S.d0 << (4 * (n - 1));
+        Wd add = 0;
+        for (int p = 0; p < n - 1; ++p) add = (add << 4) | (Wd)S.d0;
+        Wd sfx = 0, w = 0;
+        bool live = r0 < r1;
+        if (live) {
+            sfx = unrank<Wd>(r0, S.Js, n - 1, S.ks, cnt + S.cnt_off, cstride);
+            w = head | (sfx + add);
+        }
+        int64_t r = r0;
+        int qn = 0;
+        while (__any(live)) {
+            bool neck = false;
+            if (live) {
+                ++nwords;
+                bool ok = true;
+                if (P.shard_len > 0) {
+                    uint32_t idx = 0;
+                    for (int p = 0; p < P.shard_len; ++p) idx = idx * (uint32_t)k + (uint32_t)digit_at(w, n, p);
+                    ok = (shard_bits[idx >> 5] >> (idx & 31)) & 1u;
+                }
+                neck = ok && le_all_rotations(w, w, n, mask, 1);
+            }
+            const unsigned long long bn = __ballot(neck);
+            if (neck) queue[qn + __popcll(bn & below)] = w;
+            qn += __popcll(bn);
+            if (qn >= 64) {
+                drain(64, side);
+                const int rest = qn - 64;
+                const Wd t = lane < rest ? queue[64 + lane] : (Wd)0;
+                if (lane < rest) queue[lane] = t;
+                qn = rest;
+            }
+            if (live) {
+                ++r;
+                live = r < r1 && successor(sfx, n - 1, S.ks);
+                w = head | (sfx + add);
+            }
+        }
+        if (qn > 0) drain(qn, side);
+    }
+    if (cq > 0) psd_tile(cq);
+    wave_atomic_add(&ctl->words, nwords);
+    wave_atomic_add(&ctl->canon, ncanon);
+    wave_atomic_add(&ctl->band, nband);
+    for (int o = 32; o; o >>= 1) nties += (unsigned int)__shfl_xor((int)nties, o);
+    if (lane == 0 && nties) atomicAdd(&ctl->ties, nties);
+}
+
 __global__ void k_gen_unpack(const u128* __restrict__ surv_w, int64_t m, int n, int a0, int stride,
                              int8_t* __restrict__ words) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
@@ -447,7 +666,6 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
         }
     }
     P.shard_len = shard_len;
-    int64_t capacity = std::max<int64_t>(1 << 20, total / 4 + 1024);
     DevBuf& d_segs = ctx->buf("g_segs");
     DevBuf& d_tasks = ctx->buf("g_tasks");
     DevBuf& d_cnt = ctx->buf("g_cnt");
@@ -460,81 +678,143 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
     GP_HIP(hipMemcpyAsync(d_cnt.ensure(cnt.size() * 8), cnt.data(), cnt.size() * 8, hipMemcpyHostToDevice, st));
     if (!bits.empty())
         GP_HIP(hipMemcpyAsync(d_bits.ensure(bits.size() * 4), bits.data(), bits.size() * 4, hipMemcpyHostToDevice, st));
-    GenCtl h{};
-    ph.next("gen: enumerate");
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        P.capacity = capacity;
-        u128* d_w = static_cast<u128*>(ctx->buf("g_w").ensure((size_t)capacity * sizeof(u128)));
-        uint8_t* d_side = static_cast<uint8_t*>(ctx->buf("g_side").ensure((size_t)capacity));
-        GP_HIP(hipMemsetAsync(d_ctl.ensure(sizeof(GenCtl)), 0, sizeof(GenCtl), st));
-        GP_HIP(hipEventRecord(ctx->ev0, st));
-        const int64_t waves = std::min<int64_t>((int64_t)tasks.size(), (int64_t)ctx->num_cus * 32);
-        if (!tasks.empty()) {
-            const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
-            if (n <= 16)
-                hipLaunchKernelGGL(k_gen_enum<uint64_t>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
-                                   d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
-                                   bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side, d_ctl.as<GenCtl>());
-            else
-                hipLaunchKernelGGL(k_gen_enum<u128>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
-                                   d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
-                                   bits.empty() ? nullptr : d_bits.as<uint32_t>(), d_w, d_side, d_ctl.as<GenCtl>());
-        }
-        GP_HIP(hipGetLastError());
-        GP_HIP(hipEventRecord(ctx->ev1, st));
-        GP_HIP(hipMemcpyAsync(&h, d_ctl.p, sizeof h, hipMemcpyDeviceToHost, st));
-        GP_HIP(hipStreamSynchronize(st));
-        float ms = 0;
-        GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (stats) stats->kernel_ms += ms;
-        if ((int64_t)h.survivors <= capacity) break;
-        capacity = (int64_t)h.survivors + 1024;
-    }
-    const int64_t m = (int64_t)h.survivors;
-    if (stats) { stats->words += h.words; stats->canonical += (uint64_t)m; }
-    // PSD filter on the canonical survivors
-    ph.next("gen: psd + keys");
+    const int K = n / 2;
     const int table = ctx->tables.get(n);
+    ctx->tables.upload(st);
     const PsdTableDesc& d = ctx->tables.desc[table];
-    const int stride = 16 * d.nb;
-    int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)std::max<int64_t>(m, 1) * stride));
-    uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)std::max<int64_t>(m, 1)));
+    const double E = ctx->psd_mode == GP_PSD_SPLIT_BF16 ? split_bf16_E(d, (double)n * compress)
+                                                        : fp32_E(n, (double)n * compress);
+    const FilterThr thr = make_thr(2.0 * order - paf + 0.001, E);
+    int64_t m = 0;        // canonical (word, side) rows: the PSD decisions
+    int64_t acc_n = 0;    // accepted rows
+    unsigned int ties = 0;
+    int8_t* d_ow = nullptr;
+    uint8_t* d_ok = nullptr;
+    uint8_t* d_os = nullptr;
+    const int64_t waves = std::min<int64_t>((int64_t)tasks.size(), (int64_t)ctx->num_cus * 32);
+    const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+    const uint32_t* d_shard = bits.empty() ? nullptr : d_bits.as<uint32_t>();
+    if (ctx->psd_mode == GP_PSD_SPLIT_BF16) {
+        // one fused kernel: enumerate, group tests, PSD filter, FP64 band and keys
+        ph.next("gen: fused enumerate + psd + keys");
+        int64_t cap = std::max<int64_t>(1 << 16, total / 64 + 4096);
+        GenFuseCtl h{};
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            d_ow = ctx->dev<int8_t>("g_ow", (size_t)cap * n);
+            d_ok = ctx->dev<uint8_t>("g_keys", (size_t)cap * std::max(K, 1));
+            d_os = ctx->dev<uint8_t>("g_os", (size_t)cap);
+            GenFuseCtl* d_fc = static_cast<GenFuseCtl*>(d_ctl.ensure(sizeof(GenFuseCtl)));
+            GP_HIP(hipMemsetAsync(d_fc, 0, sizeof(GenFuseCtl), st));
+            GenFuse F{thr, a0, bound, cap};
+            const uint4* af = ctx->tables.d_frags.as<uint4>() + d.frag_off;
+            const double* t64 = ctx->tables.d_trig64.as<double>() + d.trig_off;
+            GP_HIP(hipEventRecord(ctx->ev0, st));
+            if (!tasks.empty()) {
+                const GenSeg* sg = d_segs.as<GenSeg>();
+                const GenTask* tk = d_tasks.as<GenTask>();
+                const uint64_t* cn = d_cnt.as<uint64_t>();
+                const int64_t nt = (int64_t)tasks.size();
+                if (n <= 16 && d.nb == 1)
+                    hipLaunchKernelGGL((k_gen_fused<uint64_t, 1>), grid, dim3(256), 0, st, P, F, sg, tk, nt, cn, cstride,
+                                       d_shard, af, t64, d_ow, d_ok, d_os, d_fc);
+                else if (n <= 16)
+                    hipLaunchKernelGGL((k_gen_fused<uint64_t, 2>), grid, dim3(256), 0, st, P, F, sg, tk, nt, cn, cstride,
+                                       d_shard, af, t64, d_ow, d_ok, d_os, d_fc);
+                else
+                    hipLaunchKernelGGL((k_gen_fused<u128, 2>), grid, dim3(256), 0, st, P, F, sg, tk, nt, cn, cstride,
+                                       d_shard, af, t64, d_ow, d_ok, d_os, d_fc);
+            }
+            GP_HIP(hipGetLastError());
+            GP_HIP(hipEventRecord(ctx->ev1, st));
+            GP_HIP(hipMemcpyAsync(&h, d_fc, sizeof h, hipMemcpyDeviceToHost, st));
+            GP_HIP(hipStreamSynchronize(st));
+            float ms = 0;
+            GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (stats) { stats->kernel_ms += ms; stats->psd_kernel_ms += ms; stats->psd_launches += 1; }
+            if ((int64_t)h.rows <= cap) break;
+            cap = (int64_t)h.rows + 1024;
+        }
+        m = (int64_t)h.canon;
+        acc_n = (int64_t)h.rows;
+        ties = h.ties;
+        if (stats) {
+            stats->words += h.words;
+            stats->canonical += (uint64_t)m;
+            stats->psd_decisions += (uint64_t)m;
+            stats->band_rechecks += h.band;
+        }
+    } else {
+        // plain FP32 path: canonical rows materialised, filtered, keyed
+        int64_t capacity = std::max<int64_t>(1 << 20, total / 4 + 1024);
+        GenCtl h{};
+        ph.next("gen: enumerate");
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            P.capacity = capacity;
+            u128* d_w = static_cast<u128*>(ctx->buf("g_w").ensure((size_t)capacity * sizeof(u128)));
+            uint8_t* d_side = static_cast<uint8_t*>(ctx->buf("g_side").ensure((size_t)capacity));
+            GP_HIP(hipMemsetAsync(d_ctl.ensure(sizeof(GenCtl)), 0, sizeof(GenCtl), st));
+            GP_HIP(hipEventRecord(ctx->ev0, st));
+            if (!tasks.empty()) {
+                if (n <= 16)
+                    hipLaunchKernelGGL(k_gen_enum<uint64_t>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
+                                       d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
+                                       d_shard, d_w, d_side, d_ctl.as<GenCtl>());
+                else
+                    hipLaunchKernelGGL(k_gen_enum<u128>, grid, dim3(256), 0, st, P, d_segs.as<GenSeg>(),
+                                       d_tasks.as<GenTask>(), (int64_t)tasks.size(), d_cnt.as<uint64_t>(), cstride,
+                                       d_shard, d_w, d_side, d_ctl.as<GenCtl>());
+            }
+            GP_HIP(hipGetLastError());
+            GP_HIP(hipEventRecord(ctx->ev1, st));
+            GP_HIP(hipMemcpyAsync(&h, d_ctl.p, sizeof h, hipMemcpyDeviceToHost, st));
+            GP_HIP(hipStreamSynchronize(st));
+            float ms = 0;
+            GP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (stats) stats->kernel_ms += ms;
+            if ((int64_t)h.survivors <= capacity) break;
+            capacity = (int64_t)h.survivors + 1024;
+        }
+        m = (int64_t)h.survivors;
+        if (stats) { stats->words += h.words; stats->canonical += (uint64_t)m; }
+        ph.next("gen: psd + keys");
+        const int stride = 16 * d.nb;
+        int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)std::max<int64_t>(m, 1) * stride));
+        uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)std::max<int64_t>(m, 1)));
+        if (m > 0) {
+            hipLaunchKernelGGL(k_gen_unpack, dim3(ctx->num_cus * 4), dim3(256), 0, st, ctx->buf("g_w").as<u128>(), m, n,
+                               a0, stride, d_words);
+            GP_HIP(hipGetLastError());
+            std::vector<FilterThr> tt(ctx->tables.desc.size());
+            tt[table] = thr;
+            std::vector<BatchSeg> bs{BatchSeg{table, 0, 0, m}};
+            run_psd_filter(ctx, d_words, stride, m, bs, tt, d_dec, stats);
+            d_ow = ctx->dev<int8_t>("g_ow", (size_t)m * n);
+            d_ok = ctx->dev<uint8_t>("g_keys", (size_t)m * std::max(K, 1));
+            d_os = ctx->dev<uint8_t>("g_os", (size_t)m);
+            auto* d_cnt_e = ctx->dev<unsigned long long>("g_emit_count", 1);
+            unsigned int* d_ties = ctx->dev<unsigned int>("g_ties", 1);
+            GP_HIP(hipMemsetAsync(d_ties, 0, sizeof(unsigned int), st));
+            GP_HIP(hipMemsetAsync(d_cnt_e, 0, sizeof(unsigned long long), st));
+            hipLaunchKernelGGL(k_gen_emit, dim3(ctx->num_cus * 4), dim3(256), 0, st, d_words, stride, m, n,
+                               ctx->tables.d_trig64.as<double>() + d.trig_off, d_dec, ctx->buf("g_side").as<uint8_t>(),
+                               bound, d_cnt_e, d_ow, d_ok, d_os, d_ties);
+            GP_HIP(hipGetLastError());
+            unsigned long long cnt_h = 0;
+            GP_HIP(hipMemcpyAsync(&cnt_h, d_cnt_e, sizeof cnt_h, hipMemcpyDeviceToHost, st));
+            GP_HIP(hipMemcpyAsync(&ties, d_ties, sizeof ties, hipMemcpyDeviceToHost, st));
+            GP_HIP(hipStreamSynchronize(st));
+            if ((int64_t)cnt_h > m) throw std::runtime_error("gen: emitted more rows than candidates");
+            acc_n = (int64_t)cnt_h;
+        }
+    }
+    // accepted rows in the reference's file order (n <= 15 on the device)
     bool sorted_on_device = false;  // rows already in file order, A rows first
     std::vector<int8_t> words_h;   // accepted rows only: [acc_n][n]
     std::vector<uint8_t> keys_h, side_h;
-    int64_t acc_n = 0;
-    if (m > 0) {
-        hipLaunchKernelGGL(k_gen_unpack, dim3(ctx->num_cus * 4), dim3(256), 0, st, ctx->buf("g_w").as<u128>(), m, n, a0,
-                           stride, d_words);
-        GP_HIP(hipGetLastError());
-        std::vector<FilterThr> tt(ctx->tables.desc.size());
-        const double E = ctx->psd_mode == GP_PSD_SPLIT_BF16 ? split_bf16_E(d, (double)n * compress)
-                                                            : fp32_E(n, (double)n * compress);
-        tt[table] = make_thr(2.0 * order - paf + 0.001, E);
-        std::vector<BatchSeg> bs{BatchSeg{table, 0, 0, m}};
-        run_psd_filter(ctx, d_words, stride, m, bs, tt, d_dec, stats);
-        const int K = n / 2;
-        int8_t* d_ow = ctx->dev<int8_t>("g_ow", (size_t)m * n);
-        uint8_t* d_ok = ctx->dev<uint8_t>("g_keys", (size_t)m * std::max(K, 1));
-        uint8_t* d_os = ctx->dev<uint8_t>("g_os", (size_t)m);
-        auto* d_cnt = ctx->dev<unsigned long long>("g_emit_count", 1);
-        unsigned int* d_ties = ctx->dev<unsigned int>("g_ties", 1);
-        GP_HIP(hipMemsetAsync(d_ties, 0, sizeof(unsigned int), st));
-        GP_HIP(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_gen_emit, dim3(ctx->num_cus * 4), dim3(256), 0, st, d_words, stride, m, n,
-                           ctx->tables.d_trig64.as<double>() + d.trig_off, d_dec, ctx->buf("g_side").as<uint8_t>(), bound,
-                           d_cnt, d_ow, d_ok, d_os, d_ties);
-        GP_HIP(hipGetLastError());
-        unsigned long long cnt_h = 0;
-        unsigned int ties = 0;
-        GP_HIP(hipMemcpyAsync(&cnt_h, d_cnt, sizeof cnt_h, hipMemcpyDeviceToHost, st));
-        GP_HIP(hipMemcpyAsync(&ties, d_ties, sizeof ties, hipMemcpyDeviceToHost, st));
-        GP_HIP(hipStreamSynchronize(st));
-        if ((int64_t)cnt_h > m) throw std::runtime_error("gen: emitted more rows than candidates");
-        acc_n = (int64_t)cnt_h;
-        words_h.resize((size_t)acc_n * n);
-        keys_h.resize((size_t)acc_n * K);
-        side_h.resize((size_t)acc_n);
+    words_h.resize((size_t)acc_n * n);
+    keys_h.resize((size_t)acc_n * K);
+    side_h.resize((size_t)acc_n);
+    if (acc_n > 0) {
         const int8_t* src_w = d_ow;
         const uint8_t* src_k = d_ok;
         if (acc_n > 1 && n <= 15) {
@@ -556,17 +836,14 @@ int generate_level(gp_ctx* ctx, int order, int compress, int paf, int proc_id, i
             src_k = d_k2;
             sorted_on_device = true;
         }
-        if (acc_n > 0) {
-            GP_HIP(hipMemcpyAsync(words_h.data(), src_w, words_h.size(), hipMemcpyDeviceToHost, st));
-            GP_HIP(hipMemcpyAsync(keys_h.data(), src_k, keys_h.size(), hipMemcpyDeviceToHost, st));
-            GP_HIP(hipMemcpyAsync(side_h.data(), d_os, side_h.size(), hipMemcpyDeviceToHost, st));
-            GP_HIP(hipStreamSynchronize(st));
-        }
-        if (stats) stats->tie_band += ties;
+        GP_HIP(hipMemcpyAsync(words_h.data(), src_w, words_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(keys_h.data(), src_k, keys_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipMemcpyAsync(side_h.data(), d_os, side_h.size(), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
     }
+    if (stats) stats->tie_band += ties;
     // collect A and B in the reference's file order (lexicographic order of the words)
     ph.next("gen: order");
-    const int K = n / 2;
     std::vector<int64_t> ia, ib;
     for (int64_t i = 0; i < acc_n; ++i) (side_h[i] ? ib : ia).push_back(i);
     if (sorted_on_device) {  // side_h is in emission order: only the A/B split counts are used
