@@ -123,11 +123,22 @@ inline void add_stats(gp_stats* dst, const gp_stats& src) {
     dst->psd_launches += src.psd_launches;
 }
 
-// Pinned host buffers for staging uploads, shared by host threads.
+// Pinned host buffers for staging uploads, shared by host threads.  `upload`
+// (created on first use) is recorded on the stream that copies the buffer to
+// the device; acquire() waits for it, so a buffer handed back while its copy is
+// still in flight (a batch dropped by an exception) is not overwritten early.
 struct PinnedBuf {
     void* p = nullptr;
     size_t bytes = 0;
     bool busy = false;
+    hipEvent_t upload = nullptr;
+    bool upload_pending = false;
+    // marks the buffer as read by work queued on `st` up to now
+    void record_upload(hipStream_t st) {
+        if (!upload) GP_HIP(hipEventCreateWithFlags(&upload, hipEventDisableTiming));
+        GP_HIP(hipEventRecord(upload, st));
+        upload_pending = true;
+    }
 };
 struct PinnedPool {
     std::mutex mu;
@@ -157,6 +168,10 @@ struct PinnedPool {
             pick->bytes = want;
         }
         pick->busy = true;
+        if (pick->upload_pending) {  // its last copy to the device must be done before it is refilled
+            GP_HIP(hipEventSynchronize(pick->upload));
+            pick->upload_pending = false;
+        }
         return pick;
     }
     void release(PinnedBuf* b) {
@@ -164,8 +179,13 @@ struct PinnedPool {
         if (b) b->busy = false;
     }
     ~PinnedPool() {
-        for (auto& b : bufs)
+        for (auto& b : bufs) {
+            if (b->upload) {
+                (void)hipEventSynchronize(b->upload);
+                (void)hipEventDestroy(b->upload);
+            }
             if (b->p) (void)hipHostFree(b->p);
+        }
     }
 };
 

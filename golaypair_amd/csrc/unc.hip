@@ -1281,8 +1281,10 @@ UncTabs upload_tables(gp_ctx* ctx, UncPrep& U, const std::string& name, UncBatch
         GP_HIP(hipMemcpyAsync(base, U.base, U.stage_bytes, hipMemcpyHostToDevice, ctx->copy_stream));
         GP_HIP(hipEventRecord(uploaded, ctx->copy_stream));
         GP_HIP(hipStreamWaitEvent(ctx->stream, uploaded, 0));
+        if (U.stage) U.stage->record_upload(ctx->copy_stream);
     } else {
         GP_HIP(hipMemcpyAsync(base, U.base, U.stage_bytes, hipMemcpyHostToDevice, ctx->stream));
+        if (U.stage) U.stage->record_upload(ctx->stream);
     }
     auto at = [&](int i) { return static_cast<void*>(base + U.stage_off[i]); };
     B.d_sides = static_cast<const UncSide*>(at(kStSides));
