@@ -142,53 +142,83 @@ struct BandCtl {
     unsigned int ties;    // FP64 values within 1e-9 of the threshold
 };
 
-template <int NB, int MT>
-__device__ __forceinline__ void psd_tile(const int8_t* __restrict__ words, int stride, const TileDesc td,
-                                         const uint4* __restrict__ afrag, const FilterThr th,
-                                         uint8_t* __restrict__ decision, BandCtl* ctl,
-                                         int64_t* __restrict__ band_rows, float* __restrict__ maxpsd, int lane) {
-    const int col = lane & 31, half = lane >> 5;
-    const bool valid = col < td.rows;
-    uint4 bop[NB];
-    const int8_t* w = words + (td.first_row + col) * (int64_t)stride + 8 * half;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        if (valid) {
-            uint2 v = *reinterpret_cast<const uint2*>(w + 16 * b);
-            bop[b] = pack8_bf16(v.x, v.y);
-        } else {
-            bop[b] = make_uint4(0, 0, 0, 0);
-        }
-    }
-    f32x16 acc[MT];
-#pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = f32x16{};
-    mfma_tile<NB, MT>(afrag, bop, acc, lane);
-    const float m = tile_max_psd<MT>(acc);
-    if (half == 0 && valid) {
-        uint8_t d = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
-        const int64_t row = td.first_row + col;
-        decision[row] = d;
-        if (maxpsd) maxpsd[row] = m;
-        if (d == kBand) {  // without a band list the rows keep their band code in `decision`
-            unsigned int i = atomicAdd(&ctl->count, 1u);
-            if (band_rows) band_rows[i] = row;
-        }
-    }
-}
-
-// The 32-row tiles of up to 8 row segments, passed by value: tile t of the
-// batch is tile t - tile0[s] of the segment s with tile0[s] <= t < tile0[s + 1].
+// Work items of up to 8 row segments, passed by value: an item is up to
+// kChunkTiles consecutive 32-row tiles of one segment; item i of the batch is
+// item i - item0[s] of the segment s with item0[s] <= i < item0[s + 1].
+constexpr int kChunkTiles = 8;
 struct SegTiles {
     int32_t nseg;
     int32_t table[8];
     int64_t first_row[8];
     int64_t rows[8];
-    int64_t tile0[9];
+    int64_t tile0[9];  // first item of each segment (and the item count)
 };
 
+// Fills T's items from the segments; returns the number of items.
+inline int64_t seg_items(SegTiles& T) {
+    int64_t n = 0;
+    for (int i = 0; i < T.nseg; ++i) {
+        T.tile0[i] = n;
+        n += (T.rows[i] + 32 * kChunkTiles - 1) / (32 * kChunkTiles);
+    }
+    T.tile0[T.nseg] = n;
+    return n;
+}
+
+// One item: `rows` (<= 32 kChunkTiles) consecutive rows from row0.  The table's
+// A fragments stay in registers for the item; the raw letters of tile j + 1
+// are loaded while tile j's MFMAs and decisions run.
+template <int NB, int MT>
+__device__ __forceinline__ void psd_chunk(const int8_t* __restrict__ words, int stride, int64_t row0, int rows,
+                                          const uint4* __restrict__ afrag, const FilterThr th,
+                                          uint8_t* __restrict__ decision, BandCtl* ctl,
+                                          int64_t* __restrict__ band_rows, float* __restrict__ maxpsd, int lane) {
+    const int col = lane & 31, half = lane >> 5;
+    uint4 a[MT][2 * NB];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int s2 = 0; s2 < 2 * NB; ++s2) a[t][s2] = afrag[((t * 2 * NB) + s2) * 64 + lane];
+    const int ntile = (rows + 31) / 32;
+    uint2 rc[NB], rn[NB];
+    auto load = [&](int j, uint2 (&raw)[NB]) {
+        const int r = j * 32 + col;
+        const int8_t* w = words + (row0 + r) * (int64_t)stride + 8 * half;
+#pragma unroll
+        for (int bl = 0; bl < NB; ++bl) raw[bl] = r < rows ? *reinterpret_cast<const uint2*>(w + 16 * bl) : make_uint2(0, 0);
+    };
+    load(0, rc);
+    for (int j = 0; j < ntile; ++j) {
+        if (j + 1 < ntile) load(j + 1, rn);
+        uint4 bop[NB];
+#pragma unroll
+        for (int bl = 0; bl < NB; ++bl) bop[bl] = pack8_bf16(rc[bl].x, rc[bl].y);
+        f32x16 acc[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = f32x16{};
+        mfma_tile_reg<NB, MT>(a, bop, acc);
+        const float m = tile_max_psd<MT>(acc);
+        const int r = j * 32 + col;
+        if (half == 0 && r < rows) {
+            const uint8_t d = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
+            const int64_t row = row0 + r;
+            decision[row] = d;
+            if (maxpsd) maxpsd[row] = m;
+            if (d == kBand) {  // without a band list the rows keep their band code in `decision`
+                unsigned int i = atomicAdd(&ctl->count, 1u);
+                if (band_rows) band_rows[i] = row;
+            }
+        }
+#pragma unroll
+        for (int bl = 0; bl < NB; ++bl) rc[bl] = rn[bl];
+    }
+}
+
+// Items of segments that share one operand class (NB position blocks, MT
+// M-tiles): one instantiation per class, so each sizes its registers alone.
+template <int NB, int MT>
 __global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ words, int stride,
-                                                  const SegTiles T, int64_t ntiles,
+                                                  const SegTiles T, int64_t nitems,
                                                   const PsdTableDesc* __restrict__ tabs,
                                                   const uint4* __restrict__ frags,
                                                   const FilterThr* __restrict__ thr,
@@ -197,19 +227,58 @@ __global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ wor
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t ti = wave; ti < ntiles; ti += nwaves) {
+    for (int64_t it = wave; it < nitems; it += nwaves) {
         int sg = 0;
-        while (sg + 1 < T.nseg && ti >= T.tile0[sg + 1]) ++sg;
-        const int64_t r0 = (ti - T.tile0[sg]) * 32;
-        const TileDesc td{T.table[sg], (int32_t)min<int64_t>(32, T.rows[sg] - r0), T.first_row[sg] + r0};
-        const PsdTableDesc d = tabs[td.table];
-        const uint4* af = frags + d.frag_off;
-        const FilterThr th = thr[td.table];
-        if (d.nb == 1) psd_tile<1, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
-        else if (d.nb == 2 && d.mt == 1) psd_tile<2, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
-        else if (d.nb == 3 && d.mt == 1) psd_tile<3, 1>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
-        else if (d.nb == 3) psd_tile<3, 2>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
-        else psd_tile<4, 2>(words, stride, td, af, th, decision, ctl, band_rows, maxpsd, lane);
+        while (sg + 1 < T.nseg && it >= T.tile0[sg + 1]) ++sg;
+        const int64_t r0 = (it - T.tile0[sg]) * (32 * kChunkTiles);
+        const int rows = (int)min<int64_t>(32 * kChunkTiles, T.rows[sg] - r0);
+        const int64_t row0 = T.first_row[sg] + r0;
+        const PsdTableDesc d = tabs[T.table[sg]];
+        psd_chunk<NB, MT>(words, stride, row0, rows, frags + d.frag_off, thr[T.table[sg]], decision, ctl, band_rows,
+                          maxpsd, lane);
+    }
+}
+
+// Launches the items of T, one kernel per operand class among its segments
+// (every resident wave slot of the class).
+void launch_psd_mfma(const SegTiles& all, const std::vector<PsdTableDesc>& desc, int num_cus, hipStream_t st,
+                     const int8_t* words, int stride, const PsdTableDesc* tabs, const uint4* frags,
+                     const FilterThr* thr, uint8_t* decision, BandCtl* ctl, int64_t* band_rows, float* maxpsd) {
+    auto cls = [&](int sg) {
+        const PsdTableDesc& d = desc[all.table[sg]];
+        return d.nb == 1 ? 0 : d.nb == 2 && d.mt == 1 ? 1 : d.nb == 3 && d.mt == 1 ? 2 : d.nb == 3 ? 3 : 4;
+    };
+    for (int c = 0; c < 5; ++c) {
+        SegTiles T{};
+        for (int sg = 0; sg < all.nseg; ++sg)
+            if (cls(sg) == c) {
+                T.table[T.nseg] = all.table[sg];
+                T.first_row[T.nseg] = all.first_row[sg];
+                T.rows[T.nseg] = all.rows[sg];
+                ++T.nseg;
+            }
+        if (T.nseg == 0) continue;
+        const int64_t nitems = seg_items(T);
+        if (nitems == 0) continue;
+        // resident waves per SIMD of each class at its registers (66, 82, 96, 138, 160 VGPRs)
+        static constexpr int kPerSimd[5] = {7, 5, 5, 3, 3};
+        const int per_simd = kPerSimd[c];
+        const int64_t waves = std::min<int64_t>(nitems, (int64_t)num_cus * 4 * per_simd);
+        const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+#define GP_PSD_CASE(C, NB, MT)                                                                                   \
+    case C:                                                                                                      \
+        hipLaunchKernelGGL((k_psd_mfma<NB, MT>), grid, dim3(256), 0, st, words, stride, T, nitems, tabs, frags, \
+                           thr, decision, ctl, band_rows, maxpsd);                                               \
+        break;
+        switch (c) {
+            GP_PSD_CASE(0, 1, 1)
+            GP_PSD_CASE(1, 2, 1)
+            GP_PSD_CASE(2, 3, 1)
+            GP_PSD_CASE(3, 3, 2)
+            GP_PSD_CASE(4, 4, 2)
+        }
+#undef GP_PSD_CASE
+        GP_HIP(hipGetLastError());
     }
 }
 
@@ -306,18 +375,15 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
     if (segs.size() > 8) throw ArgError("at most 8 word lengths per PSD batch");
     hipStream_t st = ctx->stream;
     ctx->tables.upload(st);
-    // tiles: computed on the device from the segments
+    // work items: cut on the device from the segments
     SegTiles T{};
     T.nseg = (int32_t)segs.size();
-    int64_t ntiles = 0;
     for (size_t i = 0; i < segs.size(); ++i) {
         T.table[i] = segs[i].table;
         T.first_row[i] = segs[i].first_row;
         T.rows[i] = segs[i].rows;
-        T.tile0[i] = ntiles;
-        ntiles += (segs[i].rows + 31) / 32;
     }
-    T.tile0[segs.size()] = ntiles;
+    const int64_t nitems = seg_items(T);
     BandCtl* ctl = static_cast<BandCtl*>(ctx->counters.ensure(sizeof(BandCtl)));
     int64_t* band = static_cast<int64_t*>(ctx->band_rows.ensure(std::max<int64_t>(nrows, 1) * sizeof(int64_t)));
     FilterThr* d_thr = static_cast<FilterThr*>(ctx->thr.ensure(thr_by_table.size() * sizeof(FilterThr) + 64));
@@ -327,12 +393,10 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
     GP_HIP(hipMemsetAsync(ctl, 0, sizeof(BandCtl), st));
     GP_HIP(hipEventRecord(ctx->ev0, st));
     if (ctx->psd_mode == GP_PSD_SPLIT_BF16) {
-        int64_t waves = std::min<int64_t>(ntiles, (int64_t)ctx->num_cus * 16);
-        int blocks = (int)std::max<int64_t>(1, (waves + 3) / 4);
-        if (ntiles > 0)
-            hipLaunchKernelGGL(k_psd_mfma, dim3(blocks), dim3(256), 0, st, d_words, stride, T, ntiles,
-                               ctx->tables.d_desc.as<PsdTableDesc>(),
-                               ctx->tables.d_frags.as<uint4>(), d_thr, d_decision, ctl, band, d_maxpsd);
+        if (nitems > 0)
+            launch_psd_mfma(T, ctx->tables.desc, ctx->num_cus, st, d_words, stride,
+                            ctx->tables.d_desc.as<PsdTableDesc>(), ctx->tables.d_frags.as<uint4>(), d_thr,
+                            d_decision, ctl, band, d_maxpsd);
     } else {
         int blocks = (int)std::min<int64_t>((nrows + 255) / 256, (int64_t)ctx->num_cus * 8);
         if (nrows > 0)
@@ -529,15 +593,13 @@ extern "C" int gp_psd_filter_batch_static(gp_ctx* ctx, const int8_t* d_words, in
         T.table[0] = 0;
         T.first_row[0] = 0;
         T.rows[0] = count;
-        T.tile0[0] = 0;
-        const int64_t ntiles = (count + 31) / 32;
-        T.tile0[1] = ntiles;
-        const int64_t waves = std::min<int64_t>(ntiles, (int64_t)ctx->num_cus * 16);
         GP_HIP(hipEventRecord(ctx->ev0, st));
-        hipLaunchKernelGGL(k_psd_mfma, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0, st,
-                           d_words, stride, T, ntiles, &h->desc, frags, &h->thr, d_pass, &h->ctl,
-                           static_cast<int64_t*>(nullptr), static_cast<float*>(nullptr));
-        GP_HIP(hipGetLastError());
+        {
+            // the one table in the scratch, at index 0 of its descriptor array
+            std::vector<PsdTableDesc> dd(1, ctx->tables.desc[ctx->tables.get(n)]);
+            launch_psd_mfma(T, dd, ctx->num_cus, st, d_words, stride, &h->desc, frags, &h->thr, d_pass, &h->ctl,
+                            nullptr, nullptr);
+        }
         GP_HIP(hipEventRecord(ctx->ev1, st));
         hipLaunchKernelGGL(k_na_band, dim3(blocks), dim3(256), 0, st, d_words, stride, count, trig64, h, d_pass);
         GP_HIP(hipGetLastError());

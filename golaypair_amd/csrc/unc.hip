@@ -515,7 +515,8 @@ struct Shared {
     uint4* sufb = nullptr;   // suffix B operands (split path)
     int8_t* ent = nullptr;   // suffix entries (FP32 path)
     // FP32 filter channels: slot q (row q % 16 of M-tile q / 16) holds k = chan[q]
-    std::vector<int> chan;
+    std::vector<int> chan;   // 0: an empty slot of the last M-tile (zero weights)
+    int lr = 8;              // registers per lane read in the last M-tile (8, 4 or 2)
     int mt() const { return ((int)chan.size() + 15) / 16; }
     int id(int j, int letter) const {
         const int b = block[(size_t)j * 256 + (letter + 128)];
@@ -587,6 +588,7 @@ void build_mid(const Shared& SH, const SideHost& S, const std::vector<int>& mp, 
         }
         for (size_t q = 0; q < SH.chan.size(); ++q) {
             const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+            if (k <= 0) continue;  // empty slot
             out[m * mt * 32 + t * 32 + row] = (float)acc[2 * (k - 1)];
             out[m * mt * 32 + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
         }
@@ -681,7 +683,7 @@ void build_w32(const Shared& SH, int set, int L, int r, int N, const std::vector
             for (int row = 0; row < 32; ++row) {
                 const size_t q = 16 * t + row % 16;
                 float v = 0.f;
-                if (q < SH.chan.size()) {
+                if (q < SH.chan.size() && SH.chan[q] > 0) {
                     const int mm = (int)(((int64_t)SH.chan[q] * p) % N);
                     v = (float)(row >= 16 ? sn[mm] : cs[mm]);
                 }
@@ -723,6 +725,7 @@ void add_blocks(Shared& SH, const SplitTable& X, int L, int N, const std::vector
             for (int d = 0; d < R; ++d)
                 for (size_t q = 0; q < SH.chan.size(); ++q) {
                     const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+                    if (k <= 0) continue;  // empty slot
                     c32[(int64_t)d * mt * 32 + t * 32 + row] = (float)c64[((int64_t)d * K + (k - 1)) * 2];
                     c32[(int64_t)d * mt * 32 + t * 32 + 16 + row] = (float)c64[((int64_t)d * K + (k - 1)) * 2 + 1];
                 }
@@ -747,7 +750,7 @@ double build_frags(const Shared& SH, int set, int L, int r, int N, const std::ve
                     const int part = st / nbs;
                     const size_t qs = 16 * t + row % 16;
                     uint16_t v = 0;
-                    if (e < nent && qs < SH.chan.size()) {
+                    if (e < nent && qs < SH.chan.size() && SH.chan[qs] > 0) {
                         const int k = SH.chan[qs];
                         const int j = sp[e / r], i = e % r;
                         const int p = j + L * i;
@@ -817,6 +820,7 @@ void add_side(Tables& T, const SideSlot& Z, const Shared& SH, const SideHost& S,
         D.fix_off = Z.fix_off;
         for (size_t q = 0; q < SH.chan.size(); ++q) {
             const int k = SH.chan[q], t = (int)q / 16, row = (int)q % 16;
+            if (k <= 0) continue;  // empty slot
             T.fix32[D.fix_off + t * 32 + row] = (float)acc[2 * (k - 1)];
             T.fix32[D.fix_off + t * 32 + 16 + row] = (float)acc[2 * (k - 1) + 1];
         }
@@ -896,7 +900,7 @@ V* upload(gp_ctx* ctx, const char* name, const std::vector<V>& v) {
 // Sides of one operand class of the split filter kernel, contiguous in table order.
 struct UncClass {
     int nbs = 0, mt = 0;
-    bool half = false;
+    int lr = 8;  // registers per lane read in the last M-tile (8, 4 or 2)
     int sid_lo = 0, sid_hi = 0;
     int64_t item_lo = 0, item_hi = 0;
 };
@@ -946,22 +950,22 @@ void unc_stage(UncPrep& U, PinnedPool& pool) {
     U.host.reset();
 }
 
-template <int NBS, int MT, bool HALF>
+template <int NBS, int MT, int LR>
 void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl, uint64_t* surv,
                      int64_t scap, uint64_t* band, int64_t bcap) {
-    hipLaunchKernelGGL((k_unc_filter<NBS, MT, HALF>), grid, dim3(256), 0, st, tb, c.sid_lo, c.sid_hi, c.item_lo,
+    hipLaunchKernelGGL((k_unc_filter<NBS, MT, LR>), grid, dim3(256), 0, st, tb, c.sid_lo, c.sid_hi, c.item_lo,
                        c.item_hi, ctl, surv, scap, band, bcap);
 }
 
 void launch_filter_class(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
                          uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
-#define GP_FILTER_CASE(NB, M, H)                 \
-    if (c.nbs == NB && c.mt == M && c.half == H) \
-        return launch_filter_t<NB, M, H>(c, grid, st, tb, ctl, surv, scap, band, bcap);
-    GP_FILTER_CASE(1, 1, false) GP_FILTER_CASE(2, 1, false) GP_FILTER_CASE(3, 1, false) GP_FILTER_CASE(4, 1, false)
-    GP_FILTER_CASE(1, 1, true) GP_FILTER_CASE(2, 1, true) GP_FILTER_CASE(3, 1, true) GP_FILTER_CASE(4, 1, true)
-    GP_FILTER_CASE(1, 2, false) GP_FILTER_CASE(2, 2, false) GP_FILTER_CASE(3, 2, false) GP_FILTER_CASE(4, 2, false)
-    GP_FILTER_CASE(1, 2, true) GP_FILTER_CASE(2, 2, true) GP_FILTER_CASE(3, 2, true) GP_FILTER_CASE(4, 2, true)
+#define GP_FILTER_CASE(NB, M, LR)                \
+    if (c.nbs == NB && c.mt == M && c.lr == LR) \
+        return launch_filter_t<NB, M, LR>(c, grid, st, tb, ctl, surv, scap, band, bcap);
+#define GP_FILTER_LR(NB, M) GP_FILTER_CASE(NB, M, 8) GP_FILTER_CASE(NB, M, 4) GP_FILTER_CASE(NB, M, 2)
+    GP_FILTER_LR(1, 1) GP_FILTER_LR(2, 1) GP_FILTER_LR(3, 1) GP_FILTER_LR(4, 1)
+    GP_FILTER_LR(1, 2) GP_FILTER_LR(2, 2) GP_FILTER_LR(3, 2) GP_FILTER_LR(4, 2)
+#undef GP_FILTER_LR
 #undef GP_FILTER_CASE
     throw ArgError("uncompress: no filter kernel for this operand class");
 }
@@ -1052,8 +1056,25 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         if (all_channels)
             for (size_t i = 0; i < sides.size(); ++i)
                 if (sides[i].leaves != 0) live.push_back(i);
+        // filter channels by slot: full M-tiles in order; in the last M-tile its
+        // c real channels go to rows (i mod lr) + 4 (i / lr), lr the fewest
+        // registers per lane (2, 4 or 8) that hold them, and the other slots hold
+        // 0 (no channel: zero weights, never read)
+        std::vector<int> real;
         for (int k = 1; k <= K; ++k)
-            if (all_channels || k % r != 0) SH.chan.push_back(k);
+            if (all_channels || k % r != 0) real.push_back(k);
+        const int mtr = ((int)real.size() + 15) / 16;
+        const int last = (int)real.size() - 16 * (mtr - 1);
+        SH.lr = last <= 4 ? 2 : last <= 8 ? 4 : 8;
+        SH.chan.assign(real.begin(), real.begin() + 16 * (mtr - 1));
+        if (SH.lr == 8) {
+            SH.chan.insert(SH.chan.end(), real.begin() + 16 * (mtr - 1), real.end());
+        } else {
+            std::vector<int> tile(8, 0);
+            for (int i = 0; i < last; ++i) tile[(i % SH.lr) + 4 * (i / SH.lr)] = real[16 * (mtr - 1) + i];
+            while (!tile.empty() && tile.back() == 0) tile.pop_back();
+            SH.chan.insert(SH.chan.end(), tile.begin(), tile.end());
+        }
         // shared per-(position, letter) blocks; per side its split and table keys
         Phase ph1("tables: splits + keys");
         if (!live.empty()) add_blocks(SH, *sides[live[0]].tab, sides[live[0]].L, N, cs, sn);
@@ -1103,7 +1124,6 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         }
         // table order: sides grouped by operand class (one kernel instantiation each)
         const int mtc = SH.mt();
-        const bool halfc = (int)SH.chan.size() <= 16 * (mtc - 1) + 8;
         auto cls = [&](size_t q) { return std::max(1, (spl[q].nent + 15) / 16); };
         std::vector<size_t> torder(live.size());
         std::iota(torder.begin(), torder.end(), 0);
@@ -1118,7 +1138,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
                 UncClass c;
                 c.nbs = nbs;
                 c.mt = mtc;
-                c.half = halfc;
+                c.lr = SH.lr;
                 c.sid_lo = c.sid_hi = (int)x;
                 c.item_lo = c.item_hi = T.items;
                 U.classes.push_back(c);
@@ -1323,7 +1343,7 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
     if (!U.fp32) {
         for (const UncClass& c : U.classes) {
             // as many waves as the class keeps resident (its occupancy hint, 4 SIMDs per CU)
-            const int cap = unc_waves_per_simd(c.nbs, c.mt, c.half);
+            const int cap = unc_waves_per_simd(c.nbs, c.mt, c.lr);
             const int per_simd = (int)std::clamp<int64_t>(tuned(ctx, "unc_waves/" + std::to_string(U.N), cap), 1, cap);
             const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 4 * per_simd);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));

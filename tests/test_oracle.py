@@ -5,6 +5,8 @@ Each check is independent of the restatement it tests:
   construct_generators            equals the affine group {i -> u i + t} (AGL or dihedral)
   is_canonical                    brute-force orbit minimum in numpy
   necklace enumeration            Burnside count; brute force over all words
+  gen job split (-j/-n)           prefix rank mod jobs at the reference's prefix length,
+                                  prenecklaces by a brute-force predicate
   PSD / filter / key text         numpy FFT; Parseval; Wiener-Khinchin with exact PAF
   check_if_pair                   numpy PAF
   gen                             brute force over all words (numpy FFT, orbit min)
@@ -12,14 +14,18 @@ Each check is independent of the restatement it tests:
   sort                            coreutils `sort | uniq` under LC_ALL=C on the gen files
   pair group (filter)             closure == parametric family of signed affine maps
   cache_filter                    brute-force orbits, lexicographically first input member
-  uncompress                      compress(expanded word) == rotated original; leaf count
+  uncompress                      every leaf in DFS order vs an itertools enumeration (best
+                                  letter, first-position dihedral representatives with a
+                                  Burnside count, split order, index layout, leaf count);
+                                  the PROC_ID / PROC_NUM job split; compress(expanded) ==
+                                  rotated original
   whole pipeline                  NOTES.md known answer v=34 -> 373 classes; NOTES.md v=26
                                   and v=72 fixtures; unique count independent of the
                                   compression route (v=16, 20, 26)
-Parity unpinned (code-following only): the uncompression rotation quirk and
-first-position representatives (uncompression.cpp:113-142) beyond the invariants
-above; FFTW's last-ulp PSD values (a direct FP64 DFT stands in; the tie-band
-counter reports any value within 1e-9 of a rounding or filter threshold).
+Parity unpinned (code-following only): the rotation amount of the in-place rotation loop
+(uncompression.cpp:127-131), i.e. which occurrence of the best letter ends up first; FFTW's
+last-ulp PSD values (a direct FP64 DFT stands in; the tie-band counter reports any value
+within 1e-9 of a rounding or filter threshold).
 """
 import itertools
 import math
