@@ -31,6 +31,7 @@ ORDER, C0, C1, PAF = 50, 5, 1, 0
 N1 = ORDER // C1
 FLOPS_PER_LEAF = 4 * N1 * (N1 // 2) + 3 * (N1 // 2)   # algorithmic: real DFT k=1..25 of n=50 + |X|^2 = 5075
 MFMA_PEAK_TFLOPS = 2500.0                             # MI355X dense BF16
+HBM_PEAK = 8e12                                       # MI355X HBM3E, bytes/s
 FP32_SLICE = (0, 64)                                  # matched c=5 pairs [0, 64) in file order
 PROFILE = "k_unc_filter_v50_golay_search.md"
 
@@ -157,9 +158,89 @@ def microbench(ctx, g, mode):
     acc, digest, _, st = ctx.psd_microbench(0x601A7E5EED, mix)
     ctx.set_mode(g.GP_PSD_SPLIT_BF16)
     n = int(mix[:, 4].sum())
+    # algorithmic bytes: n letters in and one decision out per candidate; the rows
+    # are stored at 16 bytes per block of 16 letters
+    alg = int(((mix[:, 0] + 1) * mix[:, 4]).sum())
+    stored = int(((16 * ((mix[:, 0] + 15) // 16) + 1) * mix[:, 4]).sum())
+    sec = st.psd_kernel_ms * 1e-3
     return {"candidates": n, "kernel_ms": st.psd_kernel_ms, "band_ms": st.band_kernel_ms,
-            "candidates_per_s": n / (st.psd_kernel_ms * 1e-3), "band_fraction": st.band_rechecks / n,
-            "accepted": int(acc.sum()), "digest": "%016x" % digest, "tie_band": st.tie_band}
+            "candidates_per_s": n / sec, "band_fraction": st.band_rechecks / n,
+            "accepted": int(acc.sum()), "digest": "%016x" % digest, "tie_band": st.tie_band,
+            "algorithmic_bytes": alg, "stored_bytes": stored,
+            "hbm_frac_algorithmic": alg / sec / HBM_PEAK, "hbm_frac_stored": stored / sec / HBM_PEAK}
+
+
+MB_LENGTHS = [[9, 10, 90, 0], [10, 5, 50, 0], [15, 3, 45, -2], [16, 1, 16, 0], [17, 3, 51, -2], [18, 5, 90, 0],
+              [25, 2, 50, 0], [9, 10, 90, 400]]
+
+
+def microbench_by_length(count=1 << 26):
+    """The microbenchmark's filter on `count` candidates of one word length at a
+    time (n, c, v, paf as in the mix; the last entry lifts the bound so that every
+    candidate lands in the FP64 band): filter and recheck kernel times, band
+    fraction, rows per second."""
+    import golaypair_amd as g
+    ctx = g.Context(0)
+    out = []
+    for n, c, v, paf in MB_LENGTHS:
+        mix = np.array([[n, c, v, paf, count]], np.int32)
+        ctx.psd_microbench(0x601A7E5EED, mix)
+        acc, _, _, st = ctx.psd_microbench(0x601A7E5EED, mix)
+        out.append({"n": n, "c": c, "v": v, "paf": paf, "kernel_ms": st.psd_kernel_ms, "band_ms": st.band_kernel_ms,
+                    "band_fraction": st.band_rechecks / count, "accepted_fraction": int(acc.sum()) / count,
+                    "rows_per_s": count / (st.psd_kernel_ms * 1e-3),
+                    "stored_bytes_per_s": count * (16 * ((n + 15) // 16) + 1) / (st.psd_kernel_ms * 1e-3)})
+    ctx.close()
+    return out
+
+
+def sort_scale(sizes=(1 << 20, 1 << 22, 1 << 24), n=15, letters=(-3, -1, 1, 3)):
+    """The device sort + uniq (gp_sort_candidates) on random candidate records of the
+    v=45 shape (n = 15 letters of {-3,-1,1,3}, 7 key digits 0..92), each size sorted
+    twice (the second timed): wall time of the call (host-to-device and back
+    included) and records per second; rocprofv3 --kernel-trace gives the kernels."""
+    import golaypair_amd as g
+    ctx = g.Context(0)
+    rng = np.random.default_rng(15)
+    out = []
+    for m in sizes:
+        words = rng.choice(letters, size=(m, n)).astype(np.int8)
+        keys = rng.integers(0, 93, size=(m, n // 2)).astype(np.uint8)
+        c = g.Cands(words, keys)
+        ctx.sort_candidates(c)
+        t0 = time.perf_counter()
+        s = ctx.sort_candidates(c)
+        dt = time.perf_counter() - t0
+        out.append({"records": m, "unique": len(s), "record_bytes": n + n // 2, "wall_ms": dt * 1e3,
+                    "records_per_s": m / dt})
+        print("sort %d records: %.2f ms" % (m, dt * 1e3), file=sys.stderr, flush=True)
+    ctx.close()
+    return out
+
+
+def match_scale(m=1 << 21, n=15, letters=(-3, -1, 1, 3)):
+    """The device merge-join (gp_match) on two random candidate sets of m records
+    of the v=45 shape, half of the B keys copied from A rows so that key groups
+    meet (their random words then fail the exact PAF check): joined twice, the
+    second timed (wall time of the call, transfers included); rocprofv3
+    --kernel-trace gives the kernels."""
+    import golaypair_amd as g
+    ctx = g.Context(0)
+    rng = np.random.default_rng(45)
+    wa = rng.choice(letters, size=(m, n)).astype(np.int8)
+    ka = rng.integers(0, 93, size=(m, n // 2)).astype(np.uint8)
+    wb = rng.choice(letters, size=(m, n)).astype(np.int8)
+    kb = rng.integers(0, 93, size=(m, n // 2)).astype(np.uint8)
+    kb[: m // 2] = ka[rng.integers(0, m, size=m // 2)]
+    a = ctx.sort_candidates(g.Cands(wa, ka))
+    b = ctx.sort_candidates(g.Cands(wb, kb))
+    ctx.match(45, 3, -2, a, b)
+    t0 = time.perf_counter()
+    rc, pa, _, st = ctx.match(45, 3, -2, a, b)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"records_a": len(a), "records_b": len(b), "record_bytes": n + n // 2, "pairs": len(pa),
+            "pair_checks": st.pair_checks, "wall_ms": dt * 1e3}
 
 
 def copy_probe(mib=1024, reps=4):
@@ -249,6 +330,14 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run only a plain 1 GiB copy (PMC calibration)")
+    ap.add_argument("--microbench-only", action="store_true",
+                    help="only: the mixed-length PSD microbenchmark (2^26 candidates) on both paths")
+    ap.add_argument("--microbench-by-length", action="store_true",
+                    help="only: the microbenchmark's filter on 2^26 candidates of each word length alone")
+    ap.add_argument("--sort-scale", action="store_true",
+                    help="only: the device candidate sort on 2^20 .. 2^24 random records")
+    ap.add_argument("--match-scale", action="store_true",
+                    help="only: the device merge-join on two random sets of 2^21 records")
     ap.add_argument("--e2e", nargs="*", metavar="CONFIG",
                     help="only: end-to-end CLI search time of %s (default all)" % ",".join(E2E_CONFIGS))
     ap.add_argument("--fp32-parity", type=int, metavar="ORDER", choices=(45, 50, 90),
@@ -259,6 +348,22 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.sort_scale:
+        print(json.dumps(sort_scale()))
+        return
+    if args.match_scale:
+        print(json.dumps(match_scale()))
+        return
+    if args.microbench_by_length:
+        print(json.dumps(microbench_by_length()))
+        return
+    if args.microbench_only:
+        import golaypair_amd as g
+        ctx = g.Context(0)
+        print(json.dumps({"split_bf16": microbench(ctx, g, g.GP_PSD_SPLIT_BF16),
+                          "fp32": microbench(ctx, g, g.GP_PSD_FP32)}))
+        ctx.close()
         return
     if args.e2e is not None:
         print(json.dumps(e2e(args.e2e or list(E2E_CONFIGS))))

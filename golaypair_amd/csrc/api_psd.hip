@@ -58,22 +58,26 @@ MbPerm mb_perm(uint64_t seed, uint64_t count) {
     return p;
 }
 
+// One word length of the microbenchmark: slots [slot_begin, slot_end), their
+// rows `stride` bytes apart from byte byte0 of the words.
 struct MbKind {
     int32_t n, c, alpha, table;
     int64_t slot_begin, slot_end;
     int32_t order, paf;
+    int64_t byte0;
+    int32_t stride, pad;
 };
 
 __global__ void k_mb_gen(uint64_t seed, uint64_t count, uint64_t a_inv, uint64_t b, const MbKind* __restrict__ kinds,
-                         int nkind, int8_t* __restrict__ words, int stride, int64_t* __restrict__ cand_of_slot) {
+                         int nkind, int8_t* __restrict__ words, int64_t* __restrict__ cand_of_slot) {
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < count;
          s += (uint64_t)gridDim.x * blockDim.x) {
         int k = 0;
         while (k + 1 < nkind && (int64_t)s >= kinds[k + 1].slot_begin) ++k;
         const MbKind kd = kinds[k];
         const uint64_t i = (a_inv * ((s + count - b) % count)) % count;
-        int8_t* w = words + s * (uint64_t)stride;
-        for (int j = 0; j < stride; ++j) {
+        int8_t* w = words + kd.byte0 + (int64_t)(s - kd.slot_begin) * kd.stride;
+        for (int j = 0; j < kd.stride; ++j) {
             int v = 0;
             if (j < kd.n) v = -kd.c + 2 * mb_letter(seed, i, j, kd.alpha);
             w[j] = (int8_t)v;
@@ -86,7 +90,8 @@ __global__ void k_mb_digest(uint64_t seed, uint64_t count, const MbKind* __restr
                             const uint8_t* __restrict__ decision, const int64_t* __restrict__ cand_of_slot,
                             unsigned long long* __restrict__ accepted, unsigned long long* __restrict__ digest,
                             uint8_t* __restrict__ pass_by_cand, int64_t pass_cap) {
-    unsigned long long dsum = 0;
+    // accepted slots counted per kind in registers (kinds are slot ranges, at most 8)
+    unsigned long long dsum = 0, nacc[8] = {};
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < count;
          s += (uint64_t)gridDim.x * blockDim.x) {
         const int64_t i = cand_of_slot[s];
@@ -94,12 +99,15 @@ __global__ void k_mb_digest(uint64_t seed, uint64_t count, const MbKind* __restr
         if (pass_by_cand && i < pass_cap) pass_by_cand[i] = acc ? 1 : 0;
         if (acc) {
             dsum += splitmix64((uint64_t)i ^ seed);
-            int k = 0;
-            while (k + 1 < nkind && (int64_t)s >= kinds[k + 1].slot_begin) ++k;
-            atomicAdd(&accepted[k], 1ull);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                nacc[k] += k < nkind && (int64_t)s >= kinds[k].slot_begin && (int64_t)s < kinds[k].slot_end;
         }
     }
     wave_atomic_add(digest, dsum);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < nkind) wave_atomic_add(&accepted[k], nacc[k]);
 }
 
 }  // namespace
@@ -177,18 +185,20 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
         if (count < 0 || (count > 0 && (!words || !lens || !orders || !pafs || !pass)))
             throw ArgError("gp_psd_filter_batch: null buffer");
         if (count == 0) return GP_OK;
-        // Group rows by (n, order, paf): one segment per group, rows padded to 64.
+        // Group rows by (n, order, paf): one segment per group, its rows at 16
+        // bytes per block of 16 letters (zero padded).
         std::map<std::tuple<int, int, int>, std::vector<int64_t>> groups;
         for (int64_t i = 0; i < count; ++i) {
             if (lens[i] < 2 || lens[i] > kMaxN || lens[i] > stride) throw ArgError("word length out of range");
             groups[{lens[i], orders[i], pafs[i]}].push_back(i);
         }
-        const int S = kMaxN;
-        std::vector<int8_t> packed((size_t)count * S, 0);
+        int64_t bytes = 0;
+        for (auto& [key, rows] : groups) bytes += (int64_t)rows.size() * 16 * ((std::get<0>(key) + 15) / 16);
+        std::vector<int8_t> packed((size_t)bytes, 0);
         std::vector<int64_t> row_of(count);
         std::vector<BatchSeg> segs;
         std::vector<FilterThr> thr(ctx->tables.desc.size() + groups.size());
-        int64_t r = 0;
+        int64_t r = 0, off = 0;
         std::vector<FilterThr> seg_thr;
         for (auto& [key, rows] : groups) {
             auto [n, order, paf] = key;
@@ -196,16 +206,19 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
             BatchSeg sg{};
             sg.first_row = r;
             sg.rows = (int64_t)rows.size();
+            sg.stride = 16 * ctx->tables.desc[table].nb;
+            sg.byte0 = off;
             double s1 = 0;
             for (int64_t i : rows) {
                 double a = 0;
                 for (int p = 0; p < n; ++p) {
                     int8_t v = words[i * stride + p];
-                    packed[r * S + p] = v;
+                    packed[off + p] = v;
                     a += std::abs((int)v);
                 }
                 s1 = std::max(s1, a);
                 row_of[i] = r++;
+                off += sg.stride;
             }
             const PsdTableDesc& d = ctx->tables.desc[table];
             const double th = 2.0 * order - paf + 0.001;
@@ -220,7 +233,7 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
         // run one batch per distinct threshold set.
         std::vector<uint8_t> dec((size_t)count);
         std::vector<float> mx(max_psd ? (size_t)count : 0);
-        int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)count * S));
+        int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)bytes));
         uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)count));
         float* d_max = max_psd ? static_cast<float*>(ctx->aux4.ensure((size_t)count * sizeof(float))) : nullptr;
         GP_HIP(hipMemcpyAsync(d_words, packed.data(), packed.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -236,7 +249,7 @@ int gp_psd_filter_batch(gp_ctx* ctx, const int8_t* words, int stride, const int3
                 run.push_back(segs[sj]);
                 tt[segs[sj].table] = seg_thr[sj];
             }
-            run_psd_filter(ctx, d_words, S, (int64_t)count, run, tt, d_dec, stats, d_max);
+            run_psd_filter(ctx, d_words, 0, (int64_t)count, run, tt, d_dec, stats, d_max);
             si = sj;
         }
         GP_HIP(hipMemcpyAsync(dec.data(), d_dec, (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
@@ -286,15 +299,14 @@ int gp_psd_microbench(gp_ctx* ctx, uint64_t seed, int64_t count, const int32_t* 
         GP_HIP(hipSetDevice(ctx->device));
         if (!mix || nmix <= 0 || nmix > 8 || count <= 0) throw ArgError("gp_psd_microbench: bad mix");
         int64_t total = 0;
-        int maxn = 0;
-        for (int k = 0; k < nmix; ++k) { total += mix[5 * k + 4]; maxn = std::max(maxn, (int)mix[5 * k]); }
+        for (int k = 0; k < nmix; ++k) total += mix[5 * k + 4];
         if (total != count) throw ArgError("gp_psd_microbench: shares must sum to count");
-        const int stride = 16 * ((maxn + 15) / 16);
         MbPerm p = mb_perm(seed, (uint64_t)count);
         std::vector<MbKind> kinds(nmix);
         std::vector<BatchSeg> segs;
         std::vector<FilterThr> tt;
-        int64_t acc = 0;
+        // rows of each length at 16 bytes per block of 16 letters, one region per length
+        int64_t acc = 0, bytes = 0;
         for (int k = 0; k < nmix; ++k) {
             MbKind& kd = kinds[k];
             kd.n = mix[5 * k]; kd.c = mix[5 * k + 1]; kd.order = mix[5 * k + 2]; kd.paf = mix[5 * k + 3];
@@ -303,6 +315,9 @@ int gp_psd_microbench(gp_ctx* ctx, uint64_t seed, int64_t count, const int32_t* 
             kd.slot_begin = acc;
             acc += mix[5 * k + 4];
             kd.slot_end = acc;
+            kd.stride = 16 * ctx->tables.desc[kd.table].nb;
+            kd.byte0 = bytes;
+            bytes += (kd.slot_end - kd.slot_begin) * kd.stride;
         }
         tt.resize(ctx->tables.desc.size());
         std::set<int> used;
@@ -314,10 +329,11 @@ int gp_psd_microbench(gp_ctx* ctx, uint64_t seed, int64_t count, const int32_t* 
             const double th = 2.0 * kinds[k].order - kinds[k].paf + 0.001;
             const double E = ctx->psd_mode == GP_PSD_SPLIT_BF16 ? split_bf16_E(d, s1) : fp32_E(kinds[k].n, s1);
             tt[kinds[k].table] = make_thr(th, E);
-            segs.push_back(BatchSeg{kinds[k].table, 0, kinds[k].slot_begin, kinds[k].slot_end - kinds[k].slot_begin});
+            segs.push_back(BatchSeg{kinds[k].table, kinds[k].stride, kinds[k].slot_begin,
+                                    kinds[k].slot_end - kinds[k].slot_begin, kinds[k].byte0});
         }
         hipStream_t st = ctx->stream;
-        int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)count * stride));
+        int8_t* d_words = static_cast<int8_t*>(ctx->words.ensure((size_t)bytes));
         uint8_t* d_dec = static_cast<uint8_t*>(ctx->decisions.ensure((size_t)count));
         int64_t* d_cand = static_cast<int64_t*>(ctx->aux0.ensure((size_t)count * sizeof(int64_t)));
         MbKind* d_kinds = static_cast<MbKind*>(ctx->aux1.ensure(sizeof(MbKind) * nmix));
@@ -326,9 +342,9 @@ int gp_psd_microbench(gp_ctx* ctx, uint64_t seed, int64_t count, const int32_t* 
         GP_HIP(hipMemcpyAsync(d_kinds, kinds.data(), sizeof(MbKind) * nmix, hipMemcpyHostToDevice, st));
         GP_HIP(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 9, st));
         hipLaunchKernelGGL(k_mb_gen, dim3(ctx->num_cus * 8), dim3(256), 0, st, seed, (uint64_t)count, p.a_inv, p.b,
-                           d_kinds, nmix, d_words, stride, d_cand);
+                           d_kinds, nmix, d_words, d_cand);
         GP_HIP(hipGetLastError());
-        run_psd_filter(ctx, d_words, stride, count, segs, tt, d_dec, stats);
+        run_psd_filter(ctx, d_words, 0, count, segs, tt, d_dec, stats);
         hipLaunchKernelGGL(k_mb_digest, dim3(ctx->num_cus * 4), dim3(256), 0, st, seed, (uint64_t)count, d_kinds,
                            nmix, d_dec, d_cand, d_cnt, d_cnt + 8, d_pass, pass_cap);
         GP_HIP(hipGetLastError());

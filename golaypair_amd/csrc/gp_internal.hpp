@@ -205,7 +205,7 @@ struct gp_ctx {
     std::string err;
     gp::PsdTables tables;
     // scratch owned by the context
-    gp::DevBuf words, decisions, band_rows, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
+    gp::DevBuf words, decisions, counters, thr, aux0, aux1, aux2, aux3, aux4, aux5;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     int num_cus = 256;
     // result of the last gp_uncompress that failed with GP_ERR_CAPACITY
@@ -251,11 +251,15 @@ FilterThr make_thr(double thr, double E);
 // most 8); the split kernel cuts them into 32-row tiles on the device.
 // Writes decisions (kReject/kAccept) after the FP64 recheck of band rows.
 // Returns PSD-kernel device time in ms; stats counts band rechecks and ties.
+// Rows [first_row, first_row + rows) of one PSD batch, all of one word length:
+// row first_row + i is at byte byte0 + i * stride of the words.  stride 0 takes
+// the batch's stride, with byte0 = first_row * that stride.
 struct BatchSeg {
     int32_t table;
-    int32_t pad;
+    int32_t stride;
     int64_t first_row;
     int64_t rows;
+    int64_t byte0;
 };
 double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nrows,
                       const std::vector<BatchSeg>& segs, const std::vector<FilterThr>& thr_by_table,

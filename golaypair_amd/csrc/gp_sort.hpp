@@ -14,12 +14,15 @@ namespace gp {
 // Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit).
 void radix_sort_pairs(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
                       uint32_t* hist, int64_t n, int begin_bit, int end_bit);
+// Bytes of the `hist` buffer for n elements (histograms and scan scratch).
 size_t radix_hist_bytes(int64_t n);
 // The same with one 8-bit pass per entry of shifts[0..nshift), least significant first.
 void radix_sort_pairs_shifts(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys, uint32_t* tmp_vals,
                              uint32_t* hist, int64_t n, const int* shifts, int nshift);
-// In-place exclusive scan of m uint32 values.
-void scan_exclusive_u32(hipStream_t st, uint32_t* v, int64_t m);
+// In-place exclusive scan of m uint32 values; `sums` is scratch of
+// scan_sums_count(m) values.
+void scan_exclusive_u32(hipStream_t st, uint32_t* v, int64_t m, uint32_t* sums);
+int64_t scan_sums_count(int64_t m);
 
 // Sort + uniq of m candidate records (int8 words of n, uint8 keys of K = n/2)
 // held on the device, in the byte order of their text lines; the kept records

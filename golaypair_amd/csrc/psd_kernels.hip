@@ -5,6 +5,7 @@
 // Semantics: src/utils/fourier.cpp:25-52 (reject iff PSD_k > 2*order - paf +
 // 0.001 for some 1 <= k <= n/2).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -137,113 +138,123 @@ FilterThr make_thr(double thr, double E) {
 // ---------------------------------------------------------------------------
 // device
 
+// Counts of the FP64 recheck.  The filter kernels leave a band row's decision
+// at kBand and count nothing: the recheck finds the band rows by scanning the
+// decisions (a counter shared by every wave of the filter serialises its
+// atomics once a few tiles in a hundred hold a band row).
 struct BandCtl {
-    unsigned int count;   // band rows appended
-    unsigned int ties;    // FP64 values within 1e-9 of the threshold
+    unsigned long long count;  // band rows rechecked in FP64
+    unsigned long long ties;   // FP64 values within 1e-9 of the threshold
 };
 
-// Work items of up to 8 row segments, passed by value: an item is up to
-// kChunkTiles consecutive 32-row tiles of one segment; item i of the batch is
-// item i - item0[s] of the segment s with item0[s] <= i < item0[s + 1].
-constexpr int kChunkTiles = 8;
+// The work of one launch (one operand class): up to 8 row segments, passed by
+// value.  Its waves are shared out between the segments in proportion to their
+// 32-row tiles: wave w of segment s (wave0[s] <= w < wave0[s + 1]) filters the
+// contiguous tiles (w - wave0[s]) tpw[s] .. + tpw[s] - 1 of s, so that a wave
+// reads its table, threshold and A fragments once and streams its rows.
 struct SegTiles {
     int32_t nseg;
     int32_t table[8];
+    int32_t stride[8];    // bytes per row
+    int32_t tpw[8];       // tiles per wave
+    int32_t wave0[9];     // first wave of each segment (and the wave count)
     int64_t first_row[8];
     int64_t rows[8];
-    int64_t tile0[9];  // first item of each segment (and the item count)
+    int64_t byte0[8];     // byte offset of the segment's first row
 };
 
-// Fills T's items from the segments; returns the number of items.
-inline int64_t seg_items(SegTiles& T) {
-    int64_t n = 0;
+// Shares at most `waves` waves out between T's segments (at least one each,
+// and at least 4 tiles per wave); fills tpw and wave0, returns the waves used.
+inline int64_t seg_waves(SegTiles& T, int64_t waves) {
+    int64_t total = 0;
+    for (int i = 0; i < T.nseg; ++i) total += (T.rows[i] + 31) / 32;
+    waves = std::max<int64_t>(1, std::min<int64_t>(waves, (total + 3) / 4));
+    int64_t used = 0;
     for (int i = 0; i < T.nseg; ++i) {
-        T.tile0[i] = n;
-        n += (T.rows[i] + 32 * kChunkTiles - 1) / (32 * kChunkTiles);
+        const int64_t tiles = (T.rows[i] + 31) / 32;
+        T.wave0[i] = (int32_t)used;
+        T.tpw[i] = 1;
+        if (tiles == 0) continue;
+        const int64_t ws = std::max<int64_t>(1, waves * tiles / std::max<int64_t>(total, 1));
+        const int64_t tpw = (tiles + ws - 1) / ws;
+        T.tpw[i] = (int32_t)tpw;
+        used += (tiles + tpw - 1) / tpw;
     }
-    T.tile0[T.nseg] = n;
-    return n;
+    T.wave0[T.nseg] = (int32_t)used;
+    return used;
 }
 
-// One item: `rows` (<= 32 kChunkTiles) consecutive rows from row0.  The table's
-// A fragments stay in registers for the item; the raw letters of tile j + 1
-// are loaded while tile j's MFMAs and decisions run.
+// Rows of segments that share one operand class (NB position blocks, MT
+// M-tiles): one instantiation per class, so each sizes its registers alone.
+// A wave keeps D tiles of letters in flight (a ring refilled D tiles ahead as
+// each tile is taken): the kernel streams its rows.  Lanes read bytes
+// 16 bl + 8 (lane / 32) .. + 7 of their row: a row holds 16 NB bytes (letters
+// past n meet zero weights in the table); lanes past the segment's last row
+// read that row again and store no decision.
 template <int NB, int MT>
-__device__ __forceinline__ void psd_chunk(const int8_t* __restrict__ words, int stride, int64_t row0, int rows,
-                                          const uint4* __restrict__ afrag, const FilterThr th,
-                                          uint8_t* __restrict__ decision, BandCtl* ctl,
-                                          int64_t* __restrict__ band_rows, float* __restrict__ maxpsd, int lane) {
-    const int col = lane & 31, half = lane >> 5;
+__global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ words, const SegTiles T,
+                                                  const PsdTableDesc* __restrict__ tabs,
+                                                  const uint4* __restrict__ frags,
+                                                  const FilterThr* __restrict__ thr,
+                                                  uint8_t* __restrict__ decision, float* __restrict__ maxpsd) {
+    constexpr int D = NB == 1 ? 8 : NB == 2 ? 4 : 2;
+    const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (wave >= T.wave0[T.nseg]) return;
+    int sg = 0;
+    while (sg + 1 < T.nseg && wave >= T.wave0[sg + 1]) ++sg;
+    const int64_t rows = T.rows[sg];
+    const int64_t q0 = (int64_t)(wave - T.wave0[sg]) * T.tpw[sg];
+    const int64_t q1 = min<int64_t>(q0 + T.tpw[sg], (rows + 31) / 32);
+    if (q0 >= q1) return;
+    const int8_t* __restrict__ base = words + T.byte0[sg] + 8 * half;
+    const int64_t stride = T.stride[sg], row_first = T.first_row[sg];
+    const PsdTableDesc d = tabs[T.table[sg]];
+    const FilterThr th = thr[T.table[sg]];
     uint4 a[MT][2 * NB];
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int s2 = 0; s2 < 2 * NB; ++s2) a[t][s2] = afrag[((t * 2 * NB) + s2) * 64 + lane];
-    const int ntile = (rows + 31) / 32;
-    uint2 rc[NB], rn[NB];
-    auto load = [&](int j, uint2 (&raw)[NB]) {
-        const int r = j * 32 + col;
-        const int8_t* w = words + (row0 + r) * (int64_t)stride + 8 * half;
+        for (int s2 = 0; s2 < 2 * NB; ++s2) a[t][s2] = frags[d.frag_off + ((t * 2 * NB) + s2) * 64 + lane];
+    auto load = [&](int64_t q, uint2 (&raw)[NB]) {
+        const int8_t* w = base + min(q * 32 + col, rows - 1) * stride;
 #pragma unroll
-        for (int bl = 0; bl < NB; ++bl) raw[bl] = r < rows ? *reinterpret_cast<const uint2*>(w + 16 * bl) : make_uint2(0, 0);
+        for (int bl = 0; bl < NB; ++bl) raw[bl] = *reinterpret_cast<const uint2*>(w + 16 * bl);
     };
-    load(0, rc);
-    for (int j = 0; j < ntile; ++j) {
-        if (j + 1 < ntile) load(j + 1, rn);
-        uint4 bop[NB];
+    uint2 ring[D][NB];
 #pragma unroll
-        for (int bl = 0; bl < NB; ++bl) bop[bl] = pack8_bf16(rc[bl].x, rc[bl].y);
-        f32x16 acc[MT];
+    for (int g = 0; g < D; ++g) load(q0 + g, ring[g]);
+    for (int64_t qq = q0; qq < q1; qq += D) {
 #pragma unroll
-        for (int t = 0; t < MT; ++t) acc[t] = f32x16{};
-        mfma_tile_reg<NB, MT>(a, bop, acc);
-        const float m = tile_max_psd<MT>(acc);
-        const int r = j * 32 + col;
-        if (half == 0 && r < rows) {
-            const uint8_t d = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
-            const int64_t row = row0 + r;
-            decision[row] = d;
-            if (maxpsd) maxpsd[row] = m;
-            if (d == kBand) {  // without a band list the rows keep their band code in `decision`
-                unsigned int i = atomicAdd(&ctl->count, 1u);
-                if (band_rows) band_rows[i] = row;
+        for (int g = 0; g < D; ++g) {
+            const int64_t q = qq + g;
+            if (q >= q1) break;
+            uint4 bop[NB];
+#pragma unroll
+            for (int bl = 0; bl < NB; ++bl) bop[bl] = pack8_bf16(ring[g][bl].x, ring[g][bl].y);
+            load(q + D, ring[g]);
+            f32x16 acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] = f32x16{};
+            mfma_tile_reg<NB, MT>(a, bop, acc);
+            const float m = tile_max_psd<MT>(acc);
+            const int64_t r = q * 32 + col;
+            const bool mine = half == 0 && r < rows;
+            const uint8_t dec = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
+            const int64_t row = row_first + r;
+            if (mine) {
+                decision[row] = dec;
+                if (maxpsd) maxpsd[row] = m;
             }
         }
-#pragma unroll
-        for (int bl = 0; bl < NB; ++bl) rc[bl] = rn[bl];
     }
 }
 
-// Items of segments that share one operand class (NB position blocks, MT
-// M-tiles): one instantiation per class, so each sizes its registers alone.
-template <int NB, int MT>
-__global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ words, int stride,
-                                                  const SegTiles T, int64_t nitems,
-                                                  const PsdTableDesc* __restrict__ tabs,
-                                                  const uint4* __restrict__ frags,
-                                                  const FilterThr* __restrict__ thr,
-                                                  uint8_t* __restrict__ decision, BandCtl* ctl,
-                                                  int64_t* __restrict__ band_rows, float* __restrict__ maxpsd) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t it = wave; it < nitems; it += nwaves) {
-        int sg = 0;
-        while (sg + 1 < T.nseg && it >= T.tile0[sg + 1]) ++sg;
-        const int64_t r0 = (it - T.tile0[sg]) * (32 * kChunkTiles);
-        const int rows = (int)min<int64_t>(32 * kChunkTiles, T.rows[sg] - r0);
-        const int64_t row0 = T.first_row[sg] + r0;
-        const PsdTableDesc d = tabs[T.table[sg]];
-        psd_chunk<NB, MT>(words, stride, row0, rows, frags + d.frag_off, thr[T.table[sg]], decision, ctl, band_rows,
-                          maxpsd, lane);
-    }
-}
-
-// Launches the items of T, one kernel per operand class among its segments
-// (every resident wave slot of the class).
+// Launches the rows of T, one kernel per operand class among its segments
+// (at most every resident wave slot of the class).
 void launch_psd_mfma(const SegTiles& all, const std::vector<PsdTableDesc>& desc, int num_cus, hipStream_t st,
-                     const int8_t* words, int stride, const PsdTableDesc* tabs, const uint4* frags,
-                     const FilterThr* thr, uint8_t* decision, BandCtl* ctl, int64_t* band_rows, float* maxpsd) {
+                     const int8_t* words, const PsdTableDesc* tabs, const uint4* frags,
+                     const FilterThr* thr, uint8_t* decision, float* maxpsd) {
     auto cls = [&](int sg) {
         const PsdTableDesc& d = desc[all.table[sg]];
         return d.nb == 1 ? 0 : d.nb == 2 && d.mt == 1 ? 1 : d.nb == 3 && d.mt == 1 ? 2 : d.nb == 3 ? 3 : 4;
@@ -253,22 +264,33 @@ void launch_psd_mfma(const SegTiles& all, const std::vector<PsdTableDesc>& desc,
         for (int sg = 0; sg < all.nseg; ++sg)
             if (cls(sg) == c) {
                 T.table[T.nseg] = all.table[sg];
+                T.stride[T.nseg] = all.stride[sg];
                 T.first_row[T.nseg] = all.first_row[sg];
                 T.rows[T.nseg] = all.rows[sg];
+                T.byte0[T.nseg] = all.byte0[sg];
                 ++T.nseg;
             }
         if (T.nseg == 0) continue;
-        const int64_t nitems = seg_items(T);
-        if (nitems == 0) continue;
-        // resident waves per SIMD of each class at its registers (66, 82, 96, 138, 160 VGPRs)
-        static constexpr int kPerSimd[5] = {7, 5, 5, 3, 3};
-        const int per_simd = kPerSimd[c];
-        const int64_t waves = std::min<int64_t>(nitems, (int64_t)num_cus * 4 * per_simd);
+        // at most every resident wave slot of the class (blocks of 4 waves per CU
+        // from the kernel's register and LDS use): a larger grid would run its
+        // last waves after the first ones
+        static const void* const kFn[5] = {
+            reinterpret_cast<const void*>(&k_psd_mfma<1, 1>), reinterpret_cast<const void*>(&k_psd_mfma<2, 1>),
+            reinterpret_cast<const void*>(&k_psd_mfma<3, 1>), reinterpret_cast<const void*>(&k_psd_mfma<3, 2>),
+            reinterpret_cast<const void*>(&k_psd_mfma<4, 2>)};
+        static std::atomic<int> blocks_per_cu[5] = {};
+        if (blocks_per_cu[c].load() == 0) {
+            int nb = 0;
+            GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kFn[c], 256, 0));
+            blocks_per_cu[c].store(std::max(nb, 1));
+        }
+        const int64_t waves = seg_waves(T, (int64_t)num_cus * 4 * blocks_per_cu[c].load());
+        if (waves == 0) continue;
         const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
 #define GP_PSD_CASE(C, NB, MT)                                                                                   \
     case C:                                                                                                      \
-        hipLaunchKernelGGL((k_psd_mfma<NB, MT>), grid, dim3(256), 0, st, words, stride, T, nitems, tabs, frags, \
-                           thr, decision, ctl, band_rows, maxpsd);                                               \
+        hipLaunchKernelGGL((k_psd_mfma<NB, MT>), grid, dim3(256), 0, st, words, T, tabs, frags, thr, decision,  \
+                           maxpsd);                                                                              \
         break;
         switch (c) {
             GP_PSD_CASE(0, 1, 1)
@@ -307,13 +329,12 @@ __device__ __forceinline__ float fp32_max_psd(const int8_t* __restrict__ w, int 
     return mx;
 }
 
-__global__ void __launch_bounds__(256) k_psd_fp32(const int8_t* __restrict__ words, int stride,
+__global__ void __launch_bounds__(256) k_psd_fp32(const int8_t* __restrict__ words,
                                                   const BatchSeg* __restrict__ segs, int nseg,
                                                   const PsdTableDesc* __restrict__ tabs,
                                                   const float* __restrict__ trig32,
                                                   const FilterThr* __restrict__ thr,
-                                                  uint8_t* __restrict__ decision, BandCtl* ctl,
-                                                  int64_t* __restrict__ band_rows, float* __restrict__ maxpsd) {
+                                                  uint8_t* __restrict__ decision, float* __restrict__ maxpsd) {
     __shared__ float lds[8][2 * kMaxN];
     __shared__ int seg_table[8];
     if (threadIdx.x < 8 && (int)threadIdx.x < nseg) seg_table[threadIdx.x] = segs[threadIdx.x].table;
@@ -333,59 +354,119 @@ __global__ void __launch_bounds__(256) k_psd_fp32(const int8_t* __restrict__ wor
         const float* s = lds[sg] + d.n;
         for (int64_t r = gid; r < S.rows; r += gsz) {
             const int64_t row = S.first_row + r;
-            const int8_t* w = words + row * (int64_t)stride;
+            const int8_t* w = words + S.byte0 + r * S.stride;
             float m = d.K <= 16 ? fp32_max_psd<16>(w, d.n, d.K, c, s) : fp32_max_psd<32>(w, d.n, d.K, c, s);
-            uint8_t dec = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
-            decision[row] = dec;
+            decision[row] = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
             if (maxpsd) maxpsd[row] = m;
-            if (dec == kBand) {
-                unsigned int i = atomicAdd(&ctl->count, 1u);
-                band_rows[i] = row;
-            }
         }
     }
 }
 
-// FP64 recheck of band rows (row -> table via segments).
-__global__ void k_psd_fp64(const int8_t* __restrict__ words, int stride, const BatchSeg* __restrict__ segs,
-                           int nseg, const PsdTableDesc* __restrict__ tabs, const double* __restrict__ trig64,
-                           const FilterThr* __restrict__ thr, uint8_t* __restrict__ decision, BandCtl* ctl,
-                           const int64_t* __restrict__ band_rows) {
-    const unsigned int nb = ctl->count;
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
-        const int64_t row = band_rows[i];
-        int sg = 0;
-        while (sg + 1 < nseg && row >= segs[sg + 1].first_row) ++sg;
+// FP64 recheck of the rows left in the band (decision kBand), segment by
+// segment.  A wave scans 512 decisions at a time (8 per lane) and settles each
+// band row it finds with all its lanes: lane q < n/2 computes PSD_{q+1} in the
+// summation order of fp64_psd_k / fp64_max_psd, from the segments' FP64
+// twiddles and the row's letters staged in LDS; the wave takes the max.
+__global__ void __launch_bounds__(256) k_psd_fp64(const int8_t* __restrict__ words,
+                                                  const BatchSeg* __restrict__ segs, int nseg,
+                                                  const PsdTableDesc* __restrict__ tabs,
+                                                  const double* __restrict__ trig64,
+                                                  const FilterThr* __restrict__ thr,
+                                                  uint8_t* __restrict__ decision, BandCtl* ctl) {
+    __shared__ double tw[8][2 * kMaxN];  // per segment: cos then sin of 2 pi m / n
+    __shared__ int8_t wl[4][kMaxN];      // per wave: the band row's letters
+    for (int sg = 0; sg < nseg && sg < 8; ++sg) {
         const PsdTableDesc d = tabs[segs[sg].table];
-        const double* c = trig64 + d.trig_off;
-        const double* s = c + d.n;
-        const double m = fp64_max_psd(words + row * (int64_t)stride, d.n, c, s);
-        const double t = thr[segs[sg].table].thr;
-        if (fabs(m - t) < 1e-9) atomicAdd(&ctl->ties, 1u);
-        decision[row] = m > t ? kReject : kAccept;
+        for (int i = threadIdx.x; i < 2 * d.n; i += blockDim.x) tw[sg][i] = trig64[d.trig_off + i];
     }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    unsigned long long nb = 0, nt = 0;
+    for (int sg = 0; sg < nseg && sg < 8; ++sg) {
+        const BatchSeg S = segs[sg];
+        const int n = tabs[S.table].n, K = n >> 1;
+        const double* c = tw[sg];
+        const double* s = tw[sg] + n;
+        const double t = thr[S.table].thr;
+        uint8_t* __restrict__ dec = decision + S.first_row;
+        for (int64_t b = wave * 512; b < S.rows; b += nwaves * 512) {
+            uint8_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t r = b + 64 * j + lane;
+                v[j] = r < S.rows ? dec[r] : (uint8_t)kAccept;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                unsigned long long bb = __ballot(v[j] == kBand);
+                while (bb) {
+                    const int64_t r = b + 64 * j + (__ffsll((long long)bb) - 1);
+                    bb &= bb - 1;
+                    if (lane < n) wl[wv][lane] = words[S.byte0 + r * S.stride + lane];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    double m = 0.0;
+                    if (lane < K) {
+                        const int k = lane + 1;
+                        double re = 0.0, im = 0.0;
+                        int q = 0;
+                        for (int p = 0; p < n; ++p) {
+                            const double x = (double)wl[wv][p];
+                            re = __builtin_fma(x, c[q], re);
+                            im = __builtin_fma(x, s[q], im);
+                            q += k;
+                            if (q >= n) q -= n;
+                        }
+                        m = re * re + im * im;
+                    }
+                    for (int o = 32; o; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+                    if (lane == 0) {
+                        ++nb;
+                        nt += fabs(m - t) < 1e-9;
+                        dec[r] = m > t ? kReject : kAccept;
+                    }
+                    __builtin_amdgcn_wave_barrier();  // wl is rewritten for the next band row
+                }
+            }
+        }
+    }
+    wave_atomic_add(&ctl->count, nb);
+    wave_atomic_add(&ctl->ties, nt);
 }
 
 // ---------------------------------------------------------------------------
 // host driver
 
 double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nrows,
-                      const std::vector<BatchSeg>& segs, const std::vector<FilterThr>& thr_by_table,
+                      const std::vector<BatchSeg>& segs_in, const std::vector<FilterThr>& thr_by_table,
                       uint8_t* d_decision, gp_stats* stats, float* d_maxpsd) {
-    if (segs.size() > 8) throw ArgError("at most 8 word lengths per PSD batch");
+    if (segs_in.size() > 8) throw ArgError("at most 8 word lengths per PSD batch");
+    std::vector<BatchSeg> segs(segs_in);
+    for (BatchSeg& sg : segs) {
+        if (sg.stride == 0) {
+            sg.stride = stride;
+            sg.byte0 = sg.first_row * stride;
+        }
+        if (sg.stride < 16 * ctx->tables.desc[sg.table].nb || sg.stride % 8 || sg.byte0 % 8)
+            throw ArgError("PSD batch: rows must be 8-byte aligned, 16 bytes per block of 16 letters");
+    }
     hipStream_t st = ctx->stream;
     ctx->tables.upload(st);
-    // work items: cut on the device from the segments
     SegTiles T{};
     T.nseg = (int32_t)segs.size();
     for (size_t i = 0; i < segs.size(); ++i) {
         T.table[i] = segs[i].table;
+        T.stride[i] = segs[i].stride;
         T.first_row[i] = segs[i].first_row;
         T.rows[i] = segs[i].rows;
+        T.byte0[i] = segs[i].byte0;
     }
-    const int64_t nitems = seg_items(T);
+    int64_t nrow_total = 0;
+    for (const BatchSeg& sg : segs) nrow_total += sg.rows;
     BandCtl* ctl = static_cast<BandCtl*>(ctx->counters.ensure(sizeof(BandCtl)));
-    int64_t* band = static_cast<int64_t*>(ctx->band_rows.ensure(std::max<int64_t>(nrows, 1) * sizeof(int64_t)));
     FilterThr* d_thr = static_cast<FilterThr*>(ctx->thr.ensure(thr_by_table.size() * sizeof(FilterThr) + 64));
     BatchSeg* d_segs = static_cast<BatchSeg*>(ctx->aux5.ensure(segs.size() * sizeof(BatchSeg) + 64));
     GP_HIP(hipMemcpyAsync(d_thr, thr_by_table.data(), thr_by_table.size() * sizeof(FilterThr), hipMemcpyHostToDevice, st));
@@ -393,22 +474,22 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
     GP_HIP(hipMemsetAsync(ctl, 0, sizeof(BandCtl), st));
     GP_HIP(hipEventRecord(ctx->ev0, st));
     if (ctx->psd_mode == GP_PSD_SPLIT_BF16) {
-        if (nitems > 0)
-            launch_psd_mfma(T, ctx->tables.desc, ctx->num_cus, st, d_words, stride,
-                            ctx->tables.d_desc.as<PsdTableDesc>(), ctx->tables.d_frags.as<uint4>(), d_thr,
-                            d_decision, ctl, band, d_maxpsd);
+        if (nrow_total > 0)
+            launch_psd_mfma(T, ctx->tables.desc, ctx->num_cus, st, d_words, ctx->tables.d_desc.as<PsdTableDesc>(),
+                            ctx->tables.d_frags.as<uint4>(), d_thr, d_decision, d_maxpsd);
     } else {
-        int blocks = (int)std::min<int64_t>((nrows + 255) / 256, (int64_t)ctx->num_cus * 8);
-        if (nrows > 0)
-            hipLaunchKernelGGL(k_psd_fp32, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_words, stride, d_segs,
+        int blocks = (int)std::min<int64_t>((nrow_total + 255) / 256, (int64_t)ctx->num_cus * 8);
+        if (nrow_total > 0)
+            hipLaunchKernelGGL(k_psd_fp32, dim3(std::max(blocks, 1)), dim3(256), 0, st, d_words, d_segs,
                                (int)segs.size(), ctx->tables.d_desc.as<PsdTableDesc>(),
-                               ctx->tables.d_trig32.as<float>(), d_thr, d_decision, ctl, band, d_maxpsd);
+                               ctx->tables.d_trig32.as<float>(), d_thr, d_decision, d_maxpsd);
     }
     GP_HIP(hipGetLastError());
     GP_HIP(hipEventRecord(ctx->ev1, st));
-    hipLaunchKernelGGL(k_psd_fp64, dim3(ctx->num_cus), dim3(256), 0, st, d_words, stride, d_segs, (int)segs.size(),
+    const int fblocks = (int)std::min<int64_t>((nrow_total + 2047) / 2048, (int64_t)ctx->num_cus * 8);
+    hipLaunchKernelGGL(k_psd_fp64, dim3(std::max(fblocks, 1)), dim3(256), 0, st, d_words, d_segs, (int)segs.size(),
                        ctx->tables.d_desc.as<PsdTableDesc>(), ctx->tables.d_trig64.as<double>(), d_thr, d_decision,
-                       ctl, band);
+                       ctl);
     GP_HIP(hipGetLastError());
     GP_HIP(hipEventRecord(ctx->ev2, st));
     BandCtl h{};
@@ -447,7 +528,7 @@ double run_psd_filter(gp_ctx* ctx, const int8_t* d_words, int stride, int64_t nr
 namespace gp {
 
 struct NaHead {
-    BandCtl ctl;                  // band rows counted by k_psd_mfma, FP64 ties
+    BandCtl ctl;                  // band rows rechecked by k_na_band, FP64 ties
     unsigned long long passed;    // rows accepted
     unsigned int s1;              // max over the rows of sum |x_p|
     unsigned int pad;
@@ -520,6 +601,7 @@ __global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ word
                                                  uint8_t* __restrict__ pass) {
     const int n = h->desc.n;
     const double t = h->thr.thr;
+    unsigned long long nb = 0, nt = 0;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < count; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = base + threadIdx.x;
         unsigned long long acc = 0;
@@ -527,7 +609,8 @@ __global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ word
             uint8_t d = pass[r];
             if (d == kBand) {
                 const double m = fp64_max_psd(words + r * stride, n, trig64, trig64 + n);
-                if (fabs(m - t) < 1e-9) atomicAdd(&h->ctl.ties, 1u);
+                ++nb;
+                nt += fabs(m - t) < 1e-9;
                 d = m > t ? kReject : kAccept;
                 pass[r] = d;
             }
@@ -535,6 +618,8 @@ __global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ word
         }
         wave_atomic_add(&h->passed, acc);
     }
+    wave_atomic_add(&h->ctl.count, nb);
+    wave_atomic_add(&h->ctl.ties, nt);
 }
 
 }  // namespace gp
@@ -591,14 +676,15 @@ extern "C" int gp_psd_filter_batch_static(gp_ctx* ctx, const int8_t* d_words, in
         SegTiles T{};
         T.nseg = 1;
         T.table[0] = 0;
+        T.stride[0] = stride;
         T.first_row[0] = 0;
         T.rows[0] = count;
+        T.byte0[0] = 0;
         GP_HIP(hipEventRecord(ctx->ev0, st));
         {
             // the one table in the scratch, at index 0 of its descriptor array
             std::vector<PsdTableDesc> dd(1, ctx->tables.desc[ctx->tables.get(n)]);
-            launch_psd_mfma(T, dd, ctx->num_cus, st, d_words, stride, &h->desc, frags, &h->thr, d_pass, &h->ctl,
-                            nullptr, nullptr);
+            launch_psd_mfma(T, dd, ctx->num_cus, st, d_words, &h->desc, frags, &h->thr, d_pass, nullptr);
         }
         GP_HIP(hipEventRecord(ctx->ev1, st));
         hipLaunchKernelGGL(k_na_band, dim3(blocks), dim3(256), 0, st, d_words, stride, count, trig64, h, d_pass);

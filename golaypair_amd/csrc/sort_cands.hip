@@ -16,8 +16,8 @@
 // prefix-free (each ends in its only space), so the lines compare as the
 // sequences of the tokens' ranks.  Equal keys are equal lines.
 //
-// Sort: LSD radix sort (k_rs_hist / k_rs_scan / k_rs_scatter, digit ranks
-// from ballots and LDS counts), least significant key word first, each word
+// Sort: LSD radix sort (k_rs_hist, a scan of the block histograms, k_rs_scatter
+// with digit ranks from ballots and LDS counts), least significant key word first, each word
 // gathered through the current permutation; only the key bytes that differ
 // between some two records get a pass (their mask is an OR / AND reduction of
 // the keys).  Uniq: a record whose key equals its predecessor's is dropped;
@@ -86,24 +86,37 @@ __global__ void __launch_bounds__(256) k_cand_keys(const int8_t* __restrict__ wo
 }
 
 // OR and AND of every key word over the records (bits that differ between some
-// two records are OR ^ AND): wave reductions, one atomic per wave and word.
+// two records are OR ^ AND): per-thread accumulation over a grid-stride loop,
+// wave and block reductions, one atomic per block and word.
 __global__ void __launch_bounds__(256) k_key_masks(const uint64_t* __restrict__ keyw, int W, int64_t m,
                                                    unsigned long long* __restrict__ or_mask,
                                                    unsigned long long* __restrict__ and_mask) {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        for (int w = 0; w < W; ++w) {
-            unsigned long long o = 0, a = ~0ull;
-            if (i < m) o = a = keyw[(int64_t)w * m + i];
-            for (int off = 32; off > 0; off >>= 1) {
-                o |= __shfl_xor(o, off, 64);
-                a &= __shfl_xor(a, off, 64);
-            }
-            if ((threadIdx.x & 63) == 0) {
-                atomicOr(&or_mask[w], o);
-                atomicAnd(&and_mask[w], a);
-            }
+    __shared__ unsigned long long so[4], sa[4];
+    for (int w = 0; w < W; ++w) {
+        unsigned long long o = 0, a = ~0ull;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+            const unsigned long long x = keyw[(int64_t)w * m + i];
+            o |= x;
+            a &= x;
         }
+        for (int off = 32; off > 0; off >>= 1) {
+            o |= __shfl_xor(o, off, 64);
+            a &= __shfl_xor(a, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            so[threadIdx.x >> 6] = o;
+            sa[threadIdx.x >> 6] = a;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+                so[0] |= so[q];
+                sa[0] &= sa[q];
+            }
+            atomicOr(&or_mask[w], so[0]);
+            atomicAnd(&and_mask[w], sa[0]);
+        }
+        __syncthreads();
     }
 }
 
@@ -147,13 +160,26 @@ __global__ void k_sort_compact(const int8_t* __restrict__ words, const uint8_t* 
     }
 }
 
+// Bit set of the letters present (bit u of word u / 32 for the byte u of the
+// letter): letters -32..31 collect in a per-lane mask, others go to LDS at once.
 __global__ void k_letters_present(const int8_t* __restrict__ words, int64_t bytes, unsigned int* __restrict__ present) {
     __shared__ unsigned int s[8];
     if (threadIdx.x < 8) s[threadIdx.x] = 0;
     __syncthreads();
+    unsigned long long mask = 0;  // bit v + 32 for letter v in [-32, 32)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes; i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned v = (uint8_t)words[i];
-        atomicOr(&s[v >> 5], 1u << (v & 31));
+        const int v = words[i];
+        if (v >= -32 && v < 32) {
+            mask |= 1ull << (v + 32);
+        } else {
+            const unsigned u = (uint8_t)v;
+            atomicOr(&s[u >> 5], 1u << (u & 31));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mask |= __shfl_xor(mask, off, 64);
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr(&s[0], (unsigned)(mask >> 32));  // letters 0..31: bytes 0..31
+        atomicOr(&s[7], (unsigned)mask);          // letters -32..-1: bytes 224..255
     }
     __syncthreads();
     if (threadIdx.x < 8 && s[threadIdx.x]) atomicOr(&present[threadIdx.x], s[threadIdx.x]);
@@ -228,7 +254,7 @@ int64_t sort_uniq_cands(gp_ctx* ctx, const int8_t* d_words, const uint8_t* d_key
     uint32_t* pos = ctx->dev<uint32_t>("s_pos", (size_t)m);
     hipLaunchKernelGGL(k_sort_uniq, dim3(blocks), dim3(256), 0, st, keyw, W, m, idx, flag, pos);
     GP_HIP(hipGetLastError());
-    scan_exclusive_u32(st, pos, m);
+    scan_exclusive_u32(st, pos, m, ctx->dev<uint32_t>("s_scan", (size_t)scan_sums_count(m)));
     hipLaunchKernelGGL(k_sort_compact, dim3(blocks), dim3(256), 0, st, d_words, d_keys, n, K, m, idx, flag, pos,
                        d_out_words, d_out_keys);
     GP_HIP(hipGetLastError());

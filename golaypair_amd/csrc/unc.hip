@@ -9,8 +9,11 @@
 // the reference emits are the PSD-filter survivors (A, B) of one input pair with
 //   PAF_A(s) + PAF_B(s) = paf c'  (s = 1..N/2)  and  sum A^2 + sum B^2 = 2v - paf + paf c'.
 // Every survivor gets a hash of its exact integer PAF vector (B: of paf c' - PAF
-// and of 2v - paf + paf c' - sum B^2); B rows go into an open-addressing hash table,
-// A rows probe it, and every candidate is verified exactly before it is emitted.
+// and of 2v - paf + paf c' - sum B^2).  In a batch on the device the rows go to
+// hash partitions, A and B apart, and each partition is joined in LDS (B rows
+// in a table, A rows probing it); the synchronous path uses one global
+// open-addressing table.  Every candidate is verified exactly before it is
+// emitted.
 // FP64 keys are computed for the emitted pairs only (their output order is the
 // text order of the key lines, match_pairs.cpp:86-93), and the FP64 key equality
 // of each emitted pair is checked (counted in tie_band when it fails).
@@ -152,12 +155,24 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
 }
 
 struct JoinCtl2 {
-    unsigned long long out;     // emitted pairs
-    unsigned long long checks;  // full PAF checks
-    unsigned long long nb;      // B rows (counted by k_unc_rows when `jctl` is given)
-    unsigned long long slots;   // table slots read by the probes
-    unsigned long long ncand;   // candidate pairs (equal hashes)
+    unsigned long long out;       // emitted pairs
+    unsigned long long checks;    // full PAF checks
+    unsigned long long nb;        // B rows (counted by k_unc_rows when `jctl` is given)
+    unsigned long long slots;     // table slots read by the probes
+    unsigned long long ncand;     // candidate pairs (equal hashes)
+    unsigned long long overflow;  // a join partition held more than kPartCap rows of one role
 };
+
+// The batched join partitions the survivors by hash: P partitions (a power of
+// two, 64 <= P <= pmax, about 512 rows or fewer each on average), each with
+// kPartCap slots for A rows and kPartCap for B rows, records
+// (hash bits 0..31 << 32) | survivor index.  The partition is hash bits 40..
+constexpr int kPartCap = 1024;
+__host__ __device__ inline int64_t part_count(int64_t n, int64_t pmax) {
+    int64_t P = 64;
+    while (P < pmax && P * 512 < n) P <<= 1;
+    return P;
+}
 
 // Clears two small device records (the join counters and the tie count of a
 // batch) with vector stores: a kernel node, so that the batch's captured graph
@@ -168,23 +183,17 @@ __global__ void k_zero_ctl(unsigned int* __restrict__ a, int na, unsigned int* _
     if (t < nb) b[t] = 0u;
 }
 
-// Hash table size in use: the host's `tmask` + 1, or, with `jctl`, the
-// smallest power of two >= 2 * (B rows) (at least 64, at most tmask + 1).
-__device__ __forceinline__ uint64_t table_mask(const JoinCtl2* jctl, uint64_t tmask) {
-    if (!jctl) return tmask;
-    const uint64_t want = 2 * jctl->nb;
-    uint64_t T = 64;
-    while (T < want && T <= tmask) T <<= 1;
-    return T - 1;
-}
-
+// Survivor rows -> PAF-vector hashes: into `hashes` (the row's role in bit 0),
+// or, with `part`, as records into the join partitions (pcnt: A then B row
+// counts per partition, zero on entry).
 __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
                            const uint64_t* __restrict__ surv, int64_t m, int paf_target, int q_target,
                            uint64_t* __restrict__ hashes, const unsigned long long* dcount, int64_t dcap,
-                           JoinCtl2* jctl) {
+                           JoinCtl2* jctl, uint64_t* __restrict__ part, uint32_t* __restrict__ pcnt, int64_t pmax) {
     int8_t w[kMaxN];
     m = list_count(dcount, m, dcap);
+    const uint64_t pmask = part ? (uint64_t)part_count(m, pmax) - 1 : 0;
     unsigned long long nb = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t rec = surv[i];
@@ -214,26 +223,73 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
             }
             h = mix_in(h, S.role ? (int64_t)paf_target - v : v);
         }
-        // the row's role in bit 0, so that insert and probe read one array
-        hashes[i] = (splitmix64(h) << 1) | (uint64_t)(S.role == 1);
+        const uint64_t hv = splitmix64(h);
+        if (part) {
+            const uint64_t q = (uint64_t)(S.role == 1) * pmax + ((hv >> 40) & pmask);
+            const uint32_t slot = atomicAdd(&pcnt[q], 1u);
+            if (slot < (uint32_t)kPartCap) part[q * kPartCap + slot] = (hv << 32) | (uint64_t)(uint32_t)i;
+            else jctl->overflow = 1ull;
+        } else {
+            // the row's role in bit 0, so that insert and probe read one array
+            hashes[i] = (hv << 1) | (uint64_t)(S.role == 1);
+        }
     }
     if (jctl) wave_atomic_add(&jctl->nb, nb);
 }
 
-constexpr uint64_t kEmpty = ~0ull;
-
-// Empties the part of the table in use (see table_mask).
-__global__ void k_ht_fill(unsigned long long* __restrict__ table, uint64_t tmask, const JoinCtl2* jctl) {
-    const uint64_t T = table_mask(jctl, tmask) + 1;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (uint64_t)gridDim.x * blockDim.x)
-        table[i] = ~0ull;
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
-// B rows (bit 0 of the hash set) go into the table, tagged with the upper hash bits.
+// Join of each partition in LDS: its B records go into an open-addressing table
+// of 2 kPartCap slots (at most half full), its A records probe it; every B row
+// with the A row's 32-bit tag in the same partition is a candidate pair,
+// checked exactly by k_pair_check (tags that collide are dropped there).
+__global__ void __launch_bounds__(256) k_part_join(const uint64_t* __restrict__ part, const uint32_t* __restrict__ pcnt,
+                                                   int64_t pmax, const unsigned long long* dcount, int64_t dcap,
+                                                   uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b,
+                                                   int64_t ccap, JoinCtl2* ctl) {
+    constexpr uint32_t T = 2 * kPartCap;
+    __shared__ unsigned long long tab[T];
+    const int64_t P = part_count(list_count(dcount, 0, dcap), pmax);
+    unsigned long long nslots = 0;
+    for (int64_t p = blockIdx.x; p < P; p += gridDim.x) {
+        for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) tab[e] = ~0ull;
+        __syncthreads();
+        const uint32_t na = min(pcnt[p], (uint32_t)kPartCap), nb = min(pcnt[pmax + p], (uint32_t)kPartCap);
+        const uint64_t* __restrict__ ra = part + p * kPartCap;
+        const uint64_t* __restrict__ rb = part + (pmax + p) * kPartCap;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+            const unsigned long long e = rb[i];
+            uint32_t s = (uint32_t)(e >> 32) & (T - 1);
+            while (atomicCAS(&tab[s], ~0ull, e) != ~0ull) s = (s + 1) & (T - 1);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) {
+            const uint64_t e = ra[i];
+            const uint32_t tag = (uint32_t)(e >> 32);
+            for (uint32_t s = tag & (T - 1);; s = (s + 1) & (T - 1)) {
+                const unsigned long long x = tab[s];
+                ++nslots;
+                if (x == ~0ull) break;
+                if ((uint32_t)(x >> 32) == tag) {
+                    const unsigned long long o = atomicAdd(&ctl->ncand, 1ull);
+                    if ((int64_t)o < ccap) { cand_a[o] = (uint32_t)e; cand_b[o] = (uint32_t)x; }
+                }
+            }
+        }
+        __syncthreads();  // the table is reset for the next partition
+    }
+    wave_atomic_add(&ctl->slots, nslots);
+}
+
+constexpr uint64_t kEmpty = ~0ull;
+
+// Global-table join (the synchronous path): B rows (bit 0 of the hash set) go
+// into the table, tagged with the upper hash bits.
 __global__ void k_ht_insert(const uint64_t* __restrict__ hashes, int64_t m, unsigned long long* __restrict__ table,
-                            uint64_t tmask, const unsigned long long* dcount, int64_t dcap, const JoinCtl2* jctl) {
-    m = list_count(dcount, m, dcap);
-    tmask = table_mask(jctl, tmask);
+                            uint64_t tmask) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t hr = hashes[i];
         if (!(hr & 1)) continue;  // B rows only
@@ -249,9 +305,7 @@ __global__ void k_ht_insert(const uint64_t* __restrict__ hashes, int64_t m, unsi
 // k_pair_check.  Hashes carry the row's role in bit 0.
 __global__ void k_ht_probe(const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
                            uint64_t tmask, uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int64_t ccap,
-                           JoinCtl2* ctl, const unsigned long long* dcount, int64_t dcap, bool device_mask) {
-    m = list_count(dcount, m, dcap);
-    tmask = table_mask(device_mask ? ctl : nullptr, tmask);
+                           JoinCtl2* ctl) {
     unsigned long long nslots = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t hr = hashes[i];
@@ -1525,11 +1579,11 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     // when more survive); the part of the table in use is sized on the device
     // from the B rows counted by k_unc_rows and stays at most half full
     F->rcap = std::min<int64_t>(F->scap, (int64_t)(U.total_leaves / 32) + (1 << 20));
-    uint64_t T = 1;
-    while (T < (uint64_t)F->rcap * 2) T <<= 1;
+    int64_t pmax = 64;
+    while (pmax * 512 < F->rcap) pmax <<= 1;
     const int q_target = B.bound + paf_target;
-    uint64_t* hashes = ctx->dev<uint64_t>("u_hash" + sfx, (size_t)F->rcap);
-    unsigned long long* table = ctx->dev<unsigned long long>("u_ht" + sfx, (size_t)T);
+    uint64_t* part = ctx->dev<uint64_t>("u_part" + sfx, (size_t)2 * pmax * kPartCap);
+    uint32_t* pcnt = ctx->dev<uint32_t>("u_pcnt" + sfx, (size_t)2 * pmax);
     F->jctl = ctx->dev<JoinCtl2>("u_jctl" + sfx, 1);
     F->ecap = ctx->unc_ecap;
     if (const char* e = std::getenv("GOLAY_UNC_EMIT_CAP"))  // fixed emitted-pair capacity (tests)
@@ -1561,16 +1615,13 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     GP_HIP(hipGetLastError());
     const unsigned long long* nsurv = &F->ctl->surv;
     const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<int64_t>((2 * pmax + 255) / 256, ctx->num_cus * 4)),
+                       dim3(256), 0, st, pcnt, 2 * pmax);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, d_surv,
-                       (int64_t)0, paf_target, q_target, hashes, nsurv, F->rcap, F->jctl);
+                       (int64_t)0, paf_target, q_target, nullptr, nsurv, F->rcap, F->jctl, part, pcnt, pmax);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_fill, dim3(ctx->num_cus * 16), dim3(256), 0, st, table, T - 1, F->jctl);
-    GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, (int64_t)0, table, T - 1, nsurv, F->rcap,
-                       F->jctl);
-    GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, hashes, (int64_t)0, table, T - 1, ca, cb, F->ccap,
-                       F->jctl, nsurv, F->rcap, true);
+    hipLaunchKernelGGL(k_part_join, dim3((unsigned)std::min<int64_t>(pmax, (int64_t)ctx->num_cus * 8)), dim3(256), 0,
+                       st, part, pcnt, pmax, nsurv, F->rcap, ca, cb, F->ccap, F->jctl);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pair_check, dim3(ctx->num_cus * 4), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup, d_surv,
                        ca, cb, F->ccap, paf_target, q_target, oa, ob, F->ecap, F->jctl);
@@ -1623,8 +1674,8 @@ UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E,
                      (unsigned long long)U.total_leaves, (unsigned long long)h.surv, hj.nb, hj.slots, ms_filter,
                      ms_band, ms_join);
     B.nsurv = (int64_t)h.surv;
-    if ((int64_t)h.surv > F.rcap || (int64_t)hj.ncand > F.ccap || (int64_t)hj.out > F.ecap) {
-        // the join's lists were too small: join again with exact sizes
+    if ((int64_t)h.surv > F.rcap || (int64_t)hj.ncand > F.ccap || (int64_t)hj.out > F.ecap || hj.overflow) {
+        // the join's lists or a partition were too small: join again with exact sizes
         ctx->unc_ecap = std::max<int64_t>(ctx->unc_ecap, 2 * (int64_t)std::max<uint64_t>(hj.out, hj.ncand / 2));
         E = unc_join(ctx, B, paf_target, stats);
         return B;
@@ -1674,10 +1725,9 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
     const unsigned grid = (unsigned)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
-                       paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr);
+                       paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr, nullptr, nullptr, (int64_t)0);
     GP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1, nullptr, (int64_t)0,
-                       nullptr);
+    hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1);
     GP_HIP(hipGetLastError());
     // candidate pairs (the list is grown once if it overflows), then the exact
     // check, whose output cannot exceed the candidates
@@ -1687,8 +1737,7 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
         uint32_t* ca = ctx->dev<uint32_t>("u_ca", (size_t)ccap);
         uint32_t* cb = ctx->dev<uint32_t>("u_cb", (size_t)ccap);
         GP_HIP(hipMemsetAsync(ctl, 0, sizeof(JoinCtl2), st));
-        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1, ca, cb, ccap, ctl,
-                           nullptr, (int64_t)0, false);
+        hipLaunchKernelGGL(k_ht_probe, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1, ca, cb, ccap, ctl);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
