@@ -104,10 +104,12 @@ __global__ void k_mb_digest(uint64_t seed, uint64_t count, const MbKind* __restr
                 nacc[k] += k < nkind && (int64_t)s >= kinds[k].slot_begin && (int64_t)s < kinds[k].slot_end;
         }
     }
-    wave_atomic_add(digest, dsum);
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (k < nkind) wave_atomic_add(&accepted[k], nacc[k]);
+    block_atomic_add(digest, dsum, nkind > 0 ? &accepted[0] : nullptr, nacc[0], nkind > 1 ? &accepted[1] : nullptr,
+                     nacc[1]);
+    block_atomic_add(nkind > 2 ? &accepted[2] : nullptr, nacc[2], nkind > 3 ? &accepted[3] : nullptr, nacc[3],
+                     nkind > 4 ? &accepted[4] : nullptr, nacc[4]);
+    block_atomic_add(nkind > 5 ? &accepted[5] : nullptr, nacc[5], nkind > 6 ? &accepted[6] : nullptr, nacc[6],
+                     nkind > 7 ? &accepted[7] : nullptr, nacc[7]);
 }
 
 }  // namespace

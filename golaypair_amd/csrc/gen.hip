@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(256) k_gen_enum(GenParams P, const GenSeg* __r
         }
         if (qn > 0) drain(qn, side);
     }
-    wave_atomic_add(&ctl->words, nwords);
+    block_atomic_add(&ctl->words, nwords);
 }
 
 // Fused gen (split-BF16 path): the enumeration and group tests of k_gen_enum,
@@ -478,9 +478,7 @@ __global__ void __launch_bounds__(256) k_gen_fused(GenParams P, GenFuse F, const
         if (qn > 0) drain(qn, side);
     }
     if (cq > 0) psd_tile(cq);
-    wave_atomic_add(&ctl->words, nwords);
-    wave_atomic_add(&ctl->canon, ncanon);
-    wave_atomic_add(&ctl->band, nband);
+    block_atomic_add(&ctl->words, nwords, &ctl->canon, ncanon, &ctl->band, nband);
     for (int o = 32; o; o >>= 1) nties += (unsigned int)__shfl_xor((int)nties, o);
     if (lane == 0 && nties) atomicAdd(&ctl->ties, nties);
 }

@@ -82,7 +82,7 @@ __global__ void k_join(const int8_t* __restrict__ wa, const uint8_t* __restrict_
             }
         }
     }
-    wave_atomic_add(&ctl->checks, checks);
+    block_atomic_add(&ctl->checks, checks);
 }
 
 JoinResult join_pairs(gp_ctx* ctx, const int8_t* d_wa, const uint8_t* d_ka, int64_t na, const int8_t* d_wb,

@@ -119,6 +119,37 @@ __device__ inline void wave_atomic_add(unsigned long long* p, unsigned long long
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(p, v);
 }
 
+// Adds the sums of v0 .. v2 over the calling block's threads to *p0 .. *p2 (a
+// null pointer skips its value) with one atomic each (none for a zero sum).
+// Every thread of the block must call it.  Counters that every wave of a
+// large grid bumps once at its end are best summed per block first: atomics on
+// one address are applied one at a time, about 10 ns apart here, so a grid of
+// thousands of waves spends tens of microseconds on them.
+__device__ inline void block_atomic_add(unsigned long long* p0, unsigned long long v0,
+                                        unsigned long long* p1 = nullptr, unsigned long long v1 = 0,
+                                        unsigned long long* p2 = nullptr, unsigned long long v2 = 0) {
+    __shared__ unsigned long long part[3][16];
+    for (int o = 32; o; o >>= 1) {
+        v0 += __shfl_xor(v0, o);
+        v1 += __shfl_xor(v1, o);
+        v2 += __shfl_xor(v2, o);
+    }
+    const int w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
+    __syncthreads();  // an earlier call may still be reading `part`
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = v0;
+        part[1][w] = v1;
+        part[2][w] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long* p = threadIdx.x == 0 ? p0 : threadIdx.x == 1 ? p1 : p2;
+        unsigned long long t = 0;
+        for (int i = 0; i < nw; ++i) t += part[threadIdx.x][i];
+        if (p && t) atomicAdd(p, t);
+    }
+}
+
 // FP64 PSD_k of one frequency k, summed in the order of fp64_psd and
 // fp64_max_psd (so all three agree bit for bit).
 __device__ inline double fp64_psd_k(const int8_t* __restrict__ w, int n, const double* __restrict__ c,

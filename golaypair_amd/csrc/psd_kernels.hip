@@ -433,8 +433,7 @@ __global__ void __launch_bounds__(256) k_psd_fp64(const int8_t* __restrict__ wor
             }
         }
     }
-    wave_atomic_add(&ctl->count, nb);
-    wave_atomic_add(&ctl->ties, nt);
+    block_atomic_add(&ctl->count, nb, &ctl->ties, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,7 +600,7 @@ __global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ word
                                                  uint8_t* __restrict__ pass) {
     const int n = h->desc.n;
     const double t = h->thr.thr;
-    unsigned long long nb = 0, nt = 0;
+    unsigned long long nb = 0, nt = 0, npass = 0;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < count; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = base + threadIdx.x;
         unsigned long long acc = 0;
@@ -616,10 +615,9 @@ __global__ void __launch_bounds__(256) k_na_band(const int8_t* __restrict__ word
             }
             acc = d == kAccept;
         }
-        wave_atomic_add(&h->passed, acc);
+        npass += acc;
     }
-    wave_atomic_add(&h->ctl.count, nb);
-    wave_atomic_add(&h->ctl.ties, nt);
+    block_atomic_add(&h->passed, npass, &h->ctl.count, nb, &h->ctl.ties, nt);
 }
 
 }  // namespace gp

@@ -118,6 +118,19 @@ __device__ __forceinline__ uint64_t mix_in(uint64_t h, int64_t v) {
     return (h ^ (uint64_t)(v + 0x100000)) * 0x100000001B3ull + 0x9E3779B97F4A7C15ull;
 }
 
+// floor(2^64 / R) + 1 for R = 2 .. kDivMagicMax.  For a 32-bit n the quotient
+// n / R is the high word of n * magic[R]: with 2^64 = R Q + rho the product is
+// n 2^64 / R + n (R - rho) / R, whose second term adds less than n / 2^64 < 1/R
+// to n / R, too little to reach the next integer.
+constexpr int kDivMagicMax = 256;
+struct DivMagic {
+    uint64_t m[kDivMagicMax + 1];
+    constexpr DivMagic() : m() {
+        for (int r = 2; r <= kDivMagicMax; ++r) m[r] = ~0ull / (uint64_t)r + 1ull;
+    }
+};
+__constant__ DivMagic kDivMagic = DivMagic();
+
 // Decodes a record into its word (N int8) and, when the side has +1 masks,
 // the word's +1 bit mask.
 __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf, const int32_t* __restrict__ radix,
@@ -129,14 +142,14 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
         int d = 0;
         if (R == 1) {
             // single-split position: digit 0, index unchanged
-        } else if (R == 3 && leaf < 0x100000000ull) {  // the ±1 letters of a 3-fold expansion
-            const uint32_t l32 = (uint32_t)leaf;
-            const uint32_t q = __umulhi(l32, 0xAAAAAAABu) >> 1;  // exact l32 / 3 for every 32-bit l32
-            d = (int)(l32 - 3u * q);
-            leaf = q;
         } else if (R == 2) {
             d = (int)(leaf & 1);
             leaf >>= 1;
+        } else if (leaf < 0x100000000ull && R <= (uint32_t)kDivMagicMax) {  // multiply-high division
+            const uint32_t l32 = (uint32_t)leaf;
+            const uint32_t q = (uint32_t)__umul64hi((uint64_t)l32, kDivMagic.m[R]);
+            d = (int)(l32 - q * R);
+            leaf = q;
         } else if (leaf < 0x100000000ull) {
             const uint32_t l32 = (uint32_t)leaf;
             d = (int)(l32 % R);
@@ -157,8 +170,6 @@ __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf,
 struct JoinCtl2 {
     unsigned long long out;       // emitted pairs
     unsigned long long checks;    // full PAF checks
-    unsigned long long nb;        // B rows (counted by k_unc_rows when `jctl` is given)
-    unsigned long long slots;     // table slots read by the probes
     unsigned long long ncand;     // candidate pairs (equal hashes)
     unsigned long long overflow;  // a join partition held more than kPartCap rows of one role
 };
@@ -194,11 +205,9 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
     int8_t w[kMaxN];
     m = list_count(dcount, m, dcap);
     const uint64_t pmask = part ? (uint64_t)part_count(m, pmax) - 1 : 0;
-    unsigned long long nb = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t rec = surv[i];
         const UncSide& S = sides[rec >> 40];
-        nb += S.role == 1;
         const int N = S.N;
         const bool fast = S.pm1_off >= 0;
         const uint64_t mask = decode_word(S, rec & ((1ull << 40) - 1), radix, tup, pm1, fast ? nullptr : w);
@@ -234,7 +243,6 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
             hashes[i] = (hv << 1) | (uint64_t)(S.role == 1);
         }
     }
-    if (jctl) wave_atomic_add(&jctl->nb, nb);
 }
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
@@ -245,43 +253,52 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
 // Join of each partition in LDS: its B records go into an open-addressing table
 // of 2 kPartCap slots (at most half full), its A records probe it; every B row
 // with the A row's 32-bit tag in the same partition is a candidate pair,
-// checked exactly by k_pair_check (tags that collide are dropped there).
+// checked exactly by k_pair_check (tags that collide are dropped there).  A
+// block loads its partition's records (kPartCap / 256 of each role per thread)
+// before it clears the table, so the loads overlap the clearing.
 __global__ void __launch_bounds__(256) k_part_join(const uint64_t* __restrict__ part, const uint32_t* __restrict__ pcnt,
                                                    int64_t pmax, const unsigned long long* dcount, int64_t dcap,
                                                    uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b,
                                                    int64_t ccap, JoinCtl2* ctl) {
     constexpr uint32_t T = 2 * kPartCap;
+    constexpr int kPer = kPartCap / 256;
     __shared__ unsigned long long tab[T];
     const int64_t P = part_count(list_count(dcount, 0, dcap), pmax);
-    unsigned long long nslots = 0;
     for (int64_t p = blockIdx.x; p < P; p += gridDim.x) {
-        for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) tab[e] = ~0ull;
-        __syncthreads();
         const uint32_t na = min(pcnt[p], (uint32_t)kPartCap), nb = min(pcnt[pmax + p], (uint32_t)kPartCap);
         const uint64_t* __restrict__ ra = part + p * kPartCap;
         const uint64_t* __restrict__ rb = part + (pmax + p) * kPartCap;
-        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
-            const unsigned long long e = rb[i];
-            uint32_t s = (uint32_t)(e >> 32) & (T - 1);
-            while (atomicCAS(&tab[s], ~0ull, e) != ~0ull) s = (s + 1) & (T - 1);
+        unsigned long long ea[kPer], eb[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const uint32_t i = q * 256 + threadIdx.x;
+            ea[q] = i < na ? ra[i] : ~0ull;
+            eb[q] = i < nb ? rb[i] : ~0ull;
+        }
+        for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) tab[e] = ~0ull;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            if (eb[q] == ~0ull) continue;
+            uint32_t s = (uint32_t)(eb[q] >> 32) & (T - 1);
+            while (atomicCAS(&tab[s], ~0ull, eb[q]) != ~0ull) s = (s + 1) & (T - 1);
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) {
-            const uint64_t e = ra[i];
-            const uint32_t tag = (uint32_t)(e >> 32);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            if (ea[q] == ~0ull) continue;
+            const uint32_t tag = (uint32_t)(ea[q] >> 32);
             for (uint32_t s = tag & (T - 1);; s = (s + 1) & (T - 1)) {
                 const unsigned long long x = tab[s];
-                ++nslots;
                 if (x == ~0ull) break;
                 if ((uint32_t)(x >> 32) == tag) {
                     const unsigned long long o = atomicAdd(&ctl->ncand, 1ull);
-                    if ((int64_t)o < ccap) { cand_a[o] = (uint32_t)e; cand_b[o] = (uint32_t)x; }
+                    if ((int64_t)o < ccap) { cand_a[o] = (uint32_t)ea[q]; cand_b[o] = (uint32_t)x; }
                 }
             }
         }
-        __syncthreads();  // the table is reset for the next partition
+        __syncthreads();  // the table is cleared for the next partition
     }
-    wave_atomic_add(&ctl->slots, nslots);
 }
 
 constexpr uint64_t kEmpty = ~0ull;
@@ -306,7 +323,6 @@ __global__ void k_ht_insert(const uint64_t* __restrict__ hashes, int64_t m, unsi
 __global__ void k_ht_probe(const uint64_t* __restrict__ hashes, int64_t m, const unsigned long long* __restrict__ table,
                            uint64_t tmask, uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int64_t ccap,
                            JoinCtl2* ctl) {
-    unsigned long long nslots = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t hr = hashes[i];
         if (hr & 1) continue;  // A rows only
@@ -314,7 +330,6 @@ __global__ void k_ht_probe(const uint64_t* __restrict__ hashes, int64_t m, const
         uint64_t slot = (h >> 24) & tmask;
         for (;;) {
             const unsigned long long e = table[slot];
-            ++nslots;
             if (e == kEmpty) break;
             if ((e >> 32) == (h & 0xFFFFFFFFull)) {
                 const uint32_t j = (uint32_t)(e & 0xFFFFFFFFull);
@@ -326,7 +341,6 @@ __global__ void k_ht_probe(const uint64_t* __restrict__ hashes, int64_t m, const
             slot = (slot + 1) & tmask;
         }
     }
-    wave_atomic_add(&ctl->slots, nslots);
 }
 
 // Exact check of the candidate pairs: sum a_p^2 + b_p^2 = q_target and
@@ -1669,10 +1683,10 @@ UncBatch unc_collect(gp_ctx* ctx, UncInflight& F, int paf_target, UncEmitted& E,
     add_filter_stats(stats, U, h, ms_filter, ms_band);
     if (std::getenv("GOLAY_PROFILE"))
         std::fprintf(stderr,
-                     "[golay] batch: %llu leaves, %llu survivors (%llu B rows, %llu slots probed); device filter "
-                     "%.3f, band %.3f, join %.3f ms\n",
-                     (unsigned long long)U.total_leaves, (unsigned long long)h.surv, hj.nb, hj.slots, ms_filter,
-                     ms_band, ms_join);
+                     "[golay] batch: %llu leaves, %llu survivors, %llu candidate pairs; device filter %.3f, band "
+                     "%.3f, join %.3f ms\n",
+                     (unsigned long long)U.total_leaves, (unsigned long long)h.surv, hj.ncand, ms_filter, ms_band,
+                     ms_join);
     B.nsurv = (int64_t)h.surv;
     if ((int64_t)h.surv > F.rcap || (int64_t)hj.ncand > F.ccap || (int64_t)hj.out > F.ecap || hj.overflow) {
         // the join's lists or a partition were too small: join again with exact sizes
