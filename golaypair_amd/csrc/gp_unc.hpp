@@ -60,6 +60,7 @@ struct UncBatch {
     const int32_t* d_pos64 = nullptr;
     uint64_t* d_surv = nullptr;      // survivors: (table side << 40) | leaf
     int64_t nsurv = 0;
+    bool all_pm1 = false;            // every side's tuple entries are +-1
 };
 
 // Emitted pairs of a batch: rows [0, count) are the A words/keys, rows

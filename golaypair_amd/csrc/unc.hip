@@ -133,36 +133,50 @@ __constant__ DivMagic kDivMagic = DivMagic();
 
 // Decodes a record into its word (N int8) and, when the side has +1 masks,
 // the word's +1 bit mask.
+// Digit of `leaf` in radix R (R >= 1); `leaf` becomes the quotient.
+__device__ __forceinline__ int next_digit(uint64_t& leaf, uint32_t R) {
+    if (R == 1) return 0;  // single-split position: digit 0, index unchanged
+    if (R == 2) {
+        const int d = (int)(leaf & 1);
+        leaf >>= 1;
+        return d;
+    }
+    if (leaf < 0x100000000ull && R <= (uint32_t)kDivMagicMax) {  // multiply-high division
+        const uint32_t l32 = (uint32_t)leaf;
+        const uint32_t q = (uint32_t)__umul64hi((uint64_t)l32, kDivMagic.m[R]);
+        leaf = q;
+        return (int)(l32 - q * R);
+    }
+    const int d = (int)(leaf % R);
+    leaf /= R;
+    return d;
+}
+
+// Positions are taken 8 at a time (last position first): their radixes load
+// together, then their digits, then their tuple and mask loads, so that a
+// survivor waits for a few memory round trips instead of two per position.
 __device__ __forceinline__ uint64_t decode_word(const UncSide& S, uint64_t leaf, const int32_t* __restrict__ radix,
                                                 const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
                                                 int8_t* w) {
     uint64_t mask = 0;
-    for (int j = S.L - 1; j >= 0; --j) {
-        const uint32_t R = (uint32_t)radix[S.radix_off + j];
-        int d = 0;
-        if (R == 1) {
-            // single-split position: digit 0, index unchanged
-        } else if (R == 2) {
-            d = (int)(leaf & 1);
-            leaf >>= 1;
-        } else if (leaf < 0x100000000ull && R <= (uint32_t)kDivMagicMax) {  // multiply-high division
-            const uint32_t l32 = (uint32_t)leaf;
-            const uint32_t q = (uint32_t)__umul64hi((uint64_t)l32, kDivMagic.m[R]);
-            d = (int)(l32 - q * R);
-            leaf = q;
-        } else if (leaf < 0x100000000ull) {
-            const uint32_t l32 = (uint32_t)leaf;
-            d = (int)(l32 % R);
-            leaf = l32 / R;
-        } else {
-            d = (int)(leaf % R);
-            leaf /= R;
+    const bool masks = pm1 && S.pm1_off >= 0;
+    for (int j0 = S.L - 1; j0 >= 0; j0 -= 8) {
+        uint32_t R[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) R[k] = j0 - k >= 0 ? (uint32_t)radix[S.radix_off + j0 - k] : 1u;
+        int d[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d[k] = next_digit(leaf, R[k]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 - k;
+            if (j < 0) break;
+            if (w) {
+                const int8_t* t = tup + S.tup_off + ((int64_t)j * S.rmax + d[k]) * S.r;
+                for (int i = 0; i < S.r; ++i) w[j + S.L * i] = t[i];
+            }
+            if (masks) mask |= pm1[S.pm1_off + (int64_t)j * S.rmax + d[k]];
         }
-        if (w) {
-            const int8_t* t = tup + S.tup_off + ((int64_t)j * S.rmax + d) * S.r;
-            for (int i = 0; i < S.r; ++i) w[j + S.L * i] = t[i];
-        }
-        if (pm1 && S.pm1_off >= 0) mask |= pm1[S.pm1_off + (int64_t)j * S.rmax + d];
     }
     return mask;
 }
@@ -197,19 +211,22 @@ __global__ void k_zero_ctl(unsigned int* __restrict__ a, int na, unsigned int* _
 // Survivor rows -> PAF-vector hashes: into `hashes` (the row's role in bit 0),
 // or, with `part`, as records into the join partitions (pcnt: A then B row
 // counts per partition, zero on entry).
+// kAllMasks: every side of the batch carries +1 masks (every tuple entry is
+// +-1), so no word is built (and the kernel needs no private array).
+template <bool kAllMasks>
 __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __restrict__ radix,
                            const int8_t* __restrict__ tup, const uint64_t* __restrict__ pm1,
                            const uint64_t* __restrict__ surv, int64_t m, int paf_target, int q_target,
                            uint64_t* __restrict__ hashes, const unsigned long long* dcount, int64_t dcap,
                            JoinCtl2* jctl, uint64_t* __restrict__ part, uint32_t* __restrict__ pcnt, int64_t pmax) {
-    int8_t w[kMaxN];
+    int8_t w[kAllMasks ? 1 : kMaxN];
     m = list_count(dcount, m, dcap);
     const uint64_t pmask = part ? (uint64_t)part_count(m, pmax) - 1 : 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t rec = surv[i];
         const UncSide& S = sides[rec >> 40];
         const int N = S.N;
-        const bool fast = S.pm1_off >= 0;
+        const bool fast = kAllMasks || S.pm1_off >= 0;
         const uint64_t mask = decode_word(S, rec & ((1ull << 40) - 1), radix, tup, pm1, fast ? nullptr : w);
         const uint64_t full = N >= 64 ? ~0ull : ((1ull << N) - 1);
         int64_t sq = 0;
@@ -978,6 +995,7 @@ struct UncClass {
 struct UncPrep {
     int N = 0, bound = 0, rmax = 1;
     bool fp32 = false;
+    bool all_pm1 = false;  // every live side's tuple entries are +-1
     int64_t trig_off = 0;
     Tables T;
     Shared SH;
@@ -1156,6 +1174,7 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
             ksuf[q] = pos_key(S, spl[q].suf);
             pm1[q] = side_pm1(S);
         });
+        U.all_pm1 = std::all_of(pm1.begin(), pm1.end(), [](char b) { return b != 0; });
         ph1.next("tables: layout");
         // content-addressed suffix weight sets (key: the suffix positions, the
         // first half of the suffix key), mid and suffix tables (built below, in
@@ -1351,6 +1370,7 @@ void batch_header(const UncPrep& U, UncBatch& B) {
     B.bound = U.bound;
     B.side_of = U.side_of;
     B.nsides_table = (int)U.T.nsides;
+    B.all_pm1 = U.all_pm1;
 }
 
 // Survivor and band list capacities for a batch of `leaves` leaves.
@@ -1415,6 +1435,22 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
             const int per_simd = (int)std::clamp<int64_t>(tuned(ctx, "unc_waves/" + std::to_string(U.N), cap), 1, cap);
             const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 4 * per_simd);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+            if (std::getenv("GOLAY_PROFILE")) {
+                // leaf-tiles per wave under the kernel's item striding: mean and heaviest wave
+                const int64_t W = (int64_t)grid.x * 4;
+                std::vector<double> load((size_t)W, 0.0);
+                double total = 0;
+                for (int sd = c.sid_lo; sd < c.sid_hi; ++sd) {
+                    const UncSide& S = U.T.sides[sd];
+                    const double cost = (double)S.p_mid * (double)(2 * ((S.ntiles + 1) / 2));
+                    for (int64_t k = 0; k < S.p_out; ++k) load[(size_t)((S.item_base + k - c.item_lo) % W)] += cost;
+                    total += cost * (double)S.p_out;
+                }
+                const double mx = *std::max_element(load.begin(), load.end());
+                std::fprintf(stderr, "[golay] filter class <%d,%d,%d>: %lld items on %lld waves, tiles per wave mean "
+                             "%.0f, max %.0f (x%.3f)\n", c.nbs, c.mt, c.lr, (long long)(c.item_hi - c.item_lo),
+                             (long long)W, total / (double)W, mx, mx * (double)W / std::max(total, 1.0));
+            }
             launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
         }
     } else {
@@ -1474,7 +1510,7 @@ UncBatch unc_launch(gp_ctx* ctx, UncPrep& U, gp_stats* stats) {
         GP_HIP(hipEventElapsedTime(&ms_filter, ctx->ev0, ctx->ev1));
         if ((int64_t)h.band > bcap) { bcap = (int64_t)h.band + 1024; continue; }
         if (h.band) {
-            hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
+            hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup,
                                ctx->tables.d_trig64.as<double>(), d_toff, d_band, ctl, d_surv, scap, (int64_t)h.band,
                                nullptr, (int64_t)0);
             GP_HIP(hipGetLastError());
@@ -1618,7 +1654,7 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     BatchGraph graph(ctx, slot);
     if (graph.on) GP_HIP(hipEventRecord(ev[2], st));  // the band time is counted with the join's
     graph.begin(st);
-    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 8), dim3(64), 0, st, B.d_sides, B.d_radix, B.d_tup,
+    hipLaunchKernelGGL(k_unc_band, dim3(ctx->num_cus * 2), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup,
                        ctx->tables.d_trig64.as<double>(), d_toff, d_band, F->ctl, d_surv, F->scap, (int64_t)0,
                        &F->ctl->band, F->bcap);
     GP_HIP(hipGetLastError());
@@ -1631,8 +1667,12 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<int64_t>((2 * pmax + 255) / 256, ctx->num_cus * 4)),
                        dim3(256), 0, st, pcnt, 2 * pmax);
-    hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, d_surv,
-                       (int64_t)0, paf_target, q_target, nullptr, nsurv, F->rcap, F->jctl, part, pcnt, pmax);
+    if (B.all_pm1)
+        hipLaunchKernelGGL(k_unc_rows<true>, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1,
+                           d_surv, (int64_t)0, paf_target, q_target, nullptr, nsurv, F->rcap, F->jctl, part, pcnt, pmax);
+    else
+        hipLaunchKernelGGL(k_unc_rows<false>, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1,
+                           d_surv, (int64_t)0, paf_target, q_target, nullptr, nsurv, F->rcap, F->jctl, part, pcnt, pmax);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_part_join, dim3((unsigned)std::min<int64_t>(pmax, (int64_t)ctx->num_cus * 8)), dim3(256), 0,
                        st, part, pcnt, pmax, nsurv, F->rcap, ca, cb, F->ccap, F->jctl);
@@ -1738,8 +1778,9 @@ UncEmitted unc_join(gp_ctx* ctx, const UncBatch& B, int paf_target, gp_stats* st
     GP_HIP(hipEventRecord(ctx->ev0, st));
     GP_HIP(hipMemsetAsync(table, 0xFF, T * sizeof(unsigned long long), st));
     const unsigned grid = (unsigned)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 16);
-    hipLaunchKernelGGL(k_unc_rows, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1, B.d_surv, m,
-                       paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr, nullptr, nullptr, (int64_t)0);
+    hipLaunchKernelGGL(k_unc_rows<false>, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1,
+                       B.d_surv, m, paf_target, q_target, hashes, nullptr, (int64_t)0, nullptr, nullptr, nullptr,
+                       (int64_t)0);
     GP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ht_insert, dim3(grid), dim3(256), 0, st, hashes, m, table, T - 1);
     GP_HIP(hipGetLastError());
