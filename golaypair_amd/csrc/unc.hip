@@ -20,6 +20,7 @@
 #include <array>
 #include <functional>
 #include <numeric>
+#include <queue>
 #include <tuple>
 #include <map>
 #include <stdexcept>
@@ -1037,17 +1038,17 @@ void unc_stage(UncPrep& U, PinnedPool& pool) {
 }
 
 template <int NBS, int MT, int LR>
-void launch_filter_t(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl, uint64_t* surv,
-                     int64_t scap, uint64_t* band, int64_t bcap) {
+void launch_filter_t(const UncClass& c, int cls, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
+                     uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
     hipLaunchKernelGGL((k_unc_filter<NBS, MT, LR>), grid, dim3(256), 0, st, tb, c.sid_lo, c.sid_hi, c.item_lo,
-                       c.item_hi, ctl, surv, scap, band, bcap);
+                       c.item_hi, cls, ctl, surv, scap, band, bcap);
 }
 
-void launch_filter_class(const UncClass& c, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
+void launch_filter_class(const UncClass& c, int cls, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
                          uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
 #define GP_FILTER_CASE(NB, M, LR)                \
     if (c.nbs == NB && c.mt == M && c.lr == LR) \
-        return launch_filter_t<NB, M, LR>(c, grid, st, tb, ctl, surv, scap, band, bcap);
+        return launch_filter_t<NB, M, LR>(c, cls, grid, st, tb, ctl, surv, scap, band, bcap);
 #define GP_FILTER_LR(NB, M) GP_FILTER_CASE(NB, M, 8) GP_FILTER_CASE(NB, M, 4) GP_FILTER_CASE(NB, M, 2)
     GP_FILTER_LR(1, 1) GP_FILTER_LR(2, 1) GP_FILTER_LR(3, 1) GP_FILTER_LR(4, 1)
     GP_FILTER_LR(1, 2) GP_FILTER_LR(2, 2) GP_FILTER_LR(3, 2) GP_FILTER_LR(4, 2)
@@ -1214,7 +1215,13 @@ std::shared_ptr<UncPrep> unc_prepare(std::span<const SideHost> sides, std::span<
         auto cls = [&](size_t q) { return std::max(1, (spl[q].nent + 15) / 16); };
         std::vector<size_t> torder(live.size());
         std::iota(torder.begin(), torder.end(), 0);
-        std::stable_sort(torder.begin(), torder.end(), [&](size_t x, size_t y) { return cls(x) < cls(y); });
+        // within a class, sides by decreasing cost of one work item (mid rows x suffix tile
+        // pairs): the kernel hands the items out in this order to the first free wave
+        auto item_cost = [&](size_t q) { return spl[q].pm * ((spl[q].ps + 63) / 64); };
+        std::stable_sort(torder.begin(), torder.end(), [&](size_t x, size_t y) {
+            if (cls(x) != cls(y)) return cls(x) < cls(y);
+            return item_cost(x) > item_cost(y);
+        });
         std::vector<SideSlot> slot(live.size());
         int64_t n_radix = 0, n_tup = 0, n_pm1 = 0, n_pos = 0, n_olist = 0, n_fix = 0;
         for (size_t x = 0; x < torder.size(); ++x) {
@@ -1429,29 +1436,40 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
                     uint64_t* d_band, int64_t bcap) {
     hipStream_t st = ctx->stream;
     if (!U.fp32) {
-        for (const UncClass& c : U.classes) {
+        if (U.classes.size() > 4) throw ArgError("uncompress: more than 4 operand classes in a batch");
+        for (size_t ci = 0; ci < U.classes.size(); ++ci) {
+            const UncClass& c = U.classes[ci];
             // as many waves as the class keeps resident (its occupancy hint, 4 SIMDs per CU)
             const int cap = unc_waves_per_simd(c.nbs, c.mt, c.lr);
             const int per_simd = (int)std::clamp<int64_t>(tuned(ctx, "unc_waves/" + std::to_string(U.N), cap), 1, cap);
             const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 4 * per_simd);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
             if (std::getenv("GOLAY_PROFILE")) {
-                // leaf-tiles per wave under the kernel's item striding: mean and heaviest wave
+                // leaf-tiles per wave as the kernel hands items out (first item by wave index,
+                // then to whichever wave is free first): mean and heaviest wave
                 const int64_t W = (int64_t)grid.x * 4;
-                std::vector<double> load((size_t)W, 0.0);
-                double total = 0;
+                std::vector<double> cost;
                 for (int sd = c.sid_lo; sd < c.sid_hi; ++sd) {
                     const UncSide& S = U.T.sides[sd];
-                    const double cost = (double)S.p_mid * (double)(2 * ((S.ntiles + 1) / 2));
-                    for (int64_t k = 0; k < S.p_out; ++k) load[(size_t)((S.item_base + k - c.item_lo) % W)] += cost;
-                    total += cost * (double)S.p_out;
+                    cost.insert(cost.end(), (size_t)S.p_out, (double)S.p_mid * (double)(2 * ((S.ntiles + 1) / 2)));
                 }
-                const double mx = *std::max_element(load.begin(), load.end());
+                std::priority_queue<double, std::vector<double>, std::greater<double>> load;
+                double total = 0, mx = 0;
+                for (size_t k = 0; k < cost.size(); ++k) {
+                    double l = cost[k];
+                    if ((int64_t)k >= W) {
+                        l += load.top();
+                        load.pop();
+                    }
+                    load.push(l);
+                    total += cost[k];
+                    mx = std::max(mx, l);
+                }
                 std::fprintf(stderr, "[golay] filter class <%d,%d,%d>: %lld items on %lld waves, tiles per wave mean "
                              "%.0f, max %.0f (x%.3f)\n", c.nbs, c.mt, c.lr, (long long)(c.item_hi - c.item_lo),
                              (long long)W, total / (double)W, mx, mx * (double)W / std::max(total, 1.0));
             }
-            launch_filter_class(c, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
+            launch_filter_class(c, (int)ci, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
         }
     } else {
         const int64_t waves = std::min<int64_t>(U.T.items, (int64_t)ctx->num_cus * 8);
