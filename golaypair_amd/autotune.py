@@ -13,6 +13,11 @@ Per workload (order, compression pair, PAF constant):
                                occupancy): candidates 1 .. 5;
   unc_leaf_budget/<N>          leaves per uncompression device batch:
                                candidates 2^27 .. 2^31;
+  unc_counter_rounds/<N>       rounds of filter work items handed out by the
+                               work counters, the rest by wave index:
+                               candidates 0 .. 4 (default: every item);
+  unc_counter_streams/<N>      work counters per filter launch: 1 .. 4
+                               (default 8, one per XCD);
 the uncompression candidates timed on a fixed slice of the level's matched
 pairs.  Each candidate runs `reps` times (minimum wall time kept) and its
 output must equal the default setting's, byte for byte; the winner replaces
@@ -33,13 +38,15 @@ DEFAULT_OUT = os.path.join(PKG, "tuning.tsv")
 
 # name -> (order, coarse compression, finer compression, paf, matched pairs [lo, hi) timed)
 WORKLOADS = {
-    "v50": (50, 5, 1, 0, (0, 96)),
+    "v50": (50, 5, 1, 0, (0, 400)),
     "v45": (45, 3, 1, -2, (0, 2500)),
     "v90": (90, 10, 5, 0, (0, 156)),
 }
 GEN_RANKS = [16, 32, 64, 128, 256]
 UNC_WAVES = [1, 2, 3, 4, 5]
 UNC_BUDGET = [1 << 27, 1 << 28, 1 << 29, 1 << 30, 1 << 31]
+UNC_COUNTER_ROUNDS = [0, 1, 2, 4]
+UNC_COUNTER_STREAMS = [1, 2, 4]
 
 
 def _digest(parts):
@@ -113,7 +120,9 @@ def tune(workloads, reps, log):
                 oa, ob, _, _, _ = ctx.uncompress(order, c0, c1, paf, pa, pb)
                 return _digest([oa, ob])
 
-            for key, cand in (("unc_waves/%d" % N, UNC_WAVES), ("unc_leaf_budget/%d" % N, UNC_BUDGET)):
+            for key, cand in (("unc_waves/%d" % N, UNC_WAVES), ("unc_leaf_budget/%d" % N, UNC_BUDGET),
+                              ("unc_counter_rounds/%d" % N, UNC_COUNTER_ROUNDS),
+                              ("unc_counter_streams/%d" % N, UNC_COUNTER_STREAMS)):
                 v, t = _sweep(ctx, key, cand, unc, reps, log)
                 if v is not None:
                     chosen[key] = v

@@ -5,6 +5,10 @@
 //   unc_waves/<N>                  filter waves per SIMD (capped by each operand
 //                                  class's occupancy)              class maximum
 //   unc_leaf_budget/<N>            leaves per uncompression device batch   2^30
+//   unc_counter_rounds/<N>         filter item rounds handed out by the work
+//                                  counters (the rounds before go by wave
+//                                  index)                                   all
+//   unc_counter_streams/<N>        work counters per filter launch (1..8)      8
 //   gen_ranks_per_lane/<n>/<k>     gen ranks per lane, k letters     heuristic
 // File format: one "key value" pair per line, '#' starts a comment.  The file
 // is GOLAY_TUNING if set, else tuning.tsv beside the library's directory.

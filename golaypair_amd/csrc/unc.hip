@@ -1038,17 +1038,17 @@ void unc_stage(UncPrep& U, PinnedPool& pool) {
 }
 
 template <int NBS, int MT, int LR>
-void launch_filter_t(const UncClass& c, int cls, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
-                     uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
+void launch_filter_t(const UncClass& c, int64_t rounds, int cls, int nsub, dim3 grid, hipStream_t st,
+                     const UncTabs& tb, UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
     hipLaunchKernelGGL((k_unc_filter<NBS, MT, LR>), grid, dim3(256), 0, st, tb, c.sid_lo, c.sid_hi, c.item_lo,
-                       c.item_hi, cls, ctl, surv, scap, band, bcap);
+                       c.item_hi, rounds, cls, nsub, ctl, surv, scap, band, bcap);
 }
 
-void launch_filter_class(const UncClass& c, int cls, dim3 grid, hipStream_t st, const UncTabs& tb, UncCtl* ctl,
-                         uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
+void launch_filter_class(const UncClass& c, int64_t rounds, int cls, int nsub, dim3 grid, hipStream_t st,
+                         const UncTabs& tb, UncCtl* ctl, uint64_t* surv, int64_t scap, uint64_t* band, int64_t bcap) {
 #define GP_FILTER_CASE(NB, M, LR)                \
     if (c.nbs == NB && c.mt == M && c.lr == LR) \
-        return launch_filter_t<NB, M, LR>(c, cls, grid, st, tb, ctl, surv, scap, band, bcap);
+        return launch_filter_t<NB, M, LR>(c, rounds, cls, nsub, grid, st, tb, ctl, surv, scap, band, bcap);
 #define GP_FILTER_LR(NB, M) GP_FILTER_CASE(NB, M, 8) GP_FILTER_CASE(NB, M, 4) GP_FILTER_CASE(NB, M, 2)
     GP_FILTER_LR(1, 1) GP_FILTER_LR(2, 1) GP_FILTER_LR(3, 1) GP_FILTER_LR(4, 1)
     GP_FILTER_LR(1, 2) GP_FILTER_LR(2, 2) GP_FILTER_LR(3, 2) GP_FILTER_LR(4, 2)
@@ -1444,32 +1444,46 @@ void launch_filters(gp_ctx* ctx, const UncPrep& U, const UncTabs& tb, UncCtl* ct
             const int per_simd = (int)std::clamp<int64_t>(tuned(ctx, "unc_waves/" + std::to_string(U.N), cap), 1, cap);
             const int64_t waves = std::min<int64_t>(c.item_hi - c.item_lo, (int64_t)ctx->num_cus * 4 * per_simd);
             const dim3 grid((unsigned)std::max<int64_t>(1, (waves + 3) / 4));
+            // items by wave index for all but the last `tail` rounds, which go out through
+            // the work counters (by default every item goes through them)
+            const int64_t W = (int64_t)grid.x * 4;
+            const int64_t tail = tuned(ctx, "unc_counter_rounds/" + std::to_string(U.N), (int64_t)1 << 40);
+            const int64_t rounds = std::max<int64_t>(0, (c.item_hi - c.item_lo) / W - tail);
+            const int64_t nsub = std::clamp<int64_t>(tuned(ctx, "unc_counter_streams/" + std::to_string(U.N),
+                                                           kSubCounters), 1, std::min<int64_t>(kSubCounters, grid.x));
             if (std::getenv("GOLAY_PROFILE")) {
-                // leaf-tiles per wave as the kernel hands items out (first item by wave index,
-                // then to whichever wave is free first): mean and heaviest wave
-                const int64_t W = (int64_t)grid.x * 4;
+                // leaf-tiles per wave as the kernel hands items out (the fixed rounds, then
+                // each counter stream's items to whichever of its waves is free first):
+                // mean and heaviest wave
                 std::vector<double> cost;
                 for (int sd = c.sid_lo; sd < c.sid_hi; ++sd) {
                     const UncSide& S = U.T.sides[sd];
                     cost.insert(cost.end(), (size_t)S.p_out, (double)S.p_mid * (double)(2 * ((S.ntiles + 1) / 2)));
                 }
-                std::priority_queue<double, std::vector<double>, std::greater<double>> load;
+                std::vector<double> fixed((size_t)W, 0.0);
+                for (int64_t k = 0; k < rounds; ++k)
+                    for (int64_t w = 0; w < W; ++w) fixed[(size_t)w] += cost[(size_t)(k * W + ((k & 1) ? W - 1 - w : w))];
                 double total = 0, mx = 0;
-                for (size_t k = 0; k < cost.size(); ++k) {
-                    double l = cost[k];
-                    if ((int64_t)k >= W) {
-                        l += load.top();
+                for (size_t k = 0; k < cost.size(); ++k) total += cost[k];
+                for (int64_t j = 0; j < nsub; ++j) {
+                    std::priority_queue<double, std::vector<double>, std::greater<double>> load;
+                    for (int64_t w = 0; w < W; ++w)
+                        if ((w / 4) % nsub == j) load.push(fixed[(size_t)w]);
+                    for (int64_t k = rounds * W + j; k < (int64_t)cost.size(); k += nsub) {
+                        const double l = cost[(size_t)k] + load.top();
+                        load.pop();
+                        load.push(l);
+                    }
+                    while (!load.empty()) {
+                        mx = std::max(mx, load.top());
                         load.pop();
                     }
-                    load.push(l);
-                    total += cost[k];
-                    mx = std::max(mx, l);
                 }
                 std::fprintf(stderr, "[golay] filter class <%d,%d,%d>: %lld items on %lld waves, tiles per wave mean "
                              "%.0f, max %.0f (x%.3f)\n", c.nbs, c.mt, c.lr, (long long)(c.item_hi - c.item_lo),
                              (long long)W, total / (double)W, mx, mx * (double)W / std::max(total, 1.0));
             }
-            launch_filter_class(c, (int)ci, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
+            launch_filter_class(c, rounds, (int)ci, (int)nsub, grid, st, tb, ctl, d_surv, scap, d_band, bcap);
         }
     } else {
         const int64_t waves = std::min<int64_t>(U.T.items, (int64_t)ctx->num_cus * 8);

@@ -1,7 +1,9 @@
 """The tuning table (golaypair_amd/csrc/tuning.hip, gp_set_tuning / gp_get_tuning):
 every setting gives the same output, only the time changes.  Non-default
-settings (one wave per SIMD, small batches, extreme ranks per lane) against the
-built-in defaults, on gen and on a slice of an uncompression level."""
+settings (one wave per SIMD, small batches, extreme ranks per lane, filter
+work items by wave index but for the last rounds, one, three or five work
+counters) against the built-in defaults, on gen and on a slice of an
+uncompression level."""
 import ctypes
 
 import numpy as np
@@ -31,10 +33,13 @@ def test_tuning_set_get_roundtrip(gpu_ctx):
 
 
 @pytest.mark.parametrize("order,c0,c1,paf,pairs,settings", [
-    (45, 3, 1, -2, (0, 300), {"unc_waves/45": 1, "unc_leaf_budget/45": 1 << 20, "gen_ranks_per_lane/15/4": 4}),
+    (45, 3, 1, -2, (0, 300), {"unc_waves/45": 1, "unc_leaf_budget/45": 1 << 20, "gen_ranks_per_lane/15/4": 4,
+                              "unc_counter_rounds/45": 0, "unc_counter_streams/45": 1}),
     # matched pairs 46..48 of v=50 emit c=1 pairs (tests/golden/slice_v50_c5_1.json)
-    (50, 5, 1, 0, (46, 49), {"unc_waves/50": 2, "unc_leaf_budget/50": 1 << 22, "gen_ranks_per_lane/10/6": 1024}),
-    (90, 10, 5, 0, (0, 40), {"unc_waves/18": 1, "unc_leaf_budget/18": 1 << 18, "gen_ranks_per_lane/9/11": 7}),
+    (50, 5, 1, 0, (46, 49), {"unc_waves/50": 2, "unc_leaf_budget/50": 1 << 22, "gen_ranks_per_lane/10/6": 1024,
+                             "unc_counter_rounds/50": 2, "unc_counter_streams/50": 3}),
+    (90, 10, 5, 0, (0, 40), {"unc_waves/18": 1, "unc_leaf_budget/18": 1 << 18, "gen_ranks_per_lane/9/11": 7,
+                             "unc_counter_rounds/18": 1, "unc_counter_streams/18": 5}),
 ])
 def test_tuned_settings_give_default_output(gpu_ctx, order, c0, c1, paf, pairs, settings):
     saved = {k: _get(gpu_ctx, k) for k in settings}

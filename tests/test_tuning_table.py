@@ -7,7 +7,7 @@ import re
 import golaypair_amd as g
 
 TABLE = os.path.join(os.path.dirname(g.__file__), "tuning.tsv")
-KEYS = [r"unc_waves/\d+", r"unc_leaf_budget/\d+", r"gen_ranks_per_lane/\d+/\d+"]
+KEYS = [r"unc_waves/\d+", r"unc_leaf_budget/\d+", r"unc_counter_rounds/\d+", r"unc_counter_streams/\d+", r"gen_ranks_per_lane/\d+/\d+"]
 
 
 def test_tuning_table_format():
@@ -19,7 +19,7 @@ def test_tuning_table_format():
             continue
         key, value = line.split()
         assert any(re.fullmatch(p, key) for p in KEYS), key
-        assert int(value) > 0
+        assert int(value) >= 0 if key.startswith("unc_counter_rounds/") else int(value) > 0
         entries += 1
     head = open(TABLE).read()
     assert "golaypair_amd.autotune" in head
