@@ -819,11 +819,20 @@ void add_blocks(Shared& SH, const SplitTable& X, int L, int N, const std::vector
 }
 
 // Writes the MFMA A fragments of set `set`; returns the largest BF16 split
-// residual |w - hi - lo| of its weights.
+// residual |w - hi - lo| of its weights.  32x32x16 layout: lane l holds
+// A[row l % 32][entry 8 (l / 32) + i] (rows 0..15 Re, 16..31 Im of the tile's
+// channel slots).  The last tile of the small-tile class (unc_small_last_tile)
+// is a 16x16x32 operand, lane l holding A[m = l % 16][k = 8 (l / 16) + i]: its
+// B operand is the 32x32 one read as 16x16x32, k = 8 g + i (g = l / 16) being
+// entry 8 (g / 2) + i of leaf 16 (g % 2) + n, so row m = 4 G + c (G = m / 4)
+// holds, for leaves 16 (G % 2) + n only (k groups g with g % 2 == G % 2), the
+// Re (c even) or Im (c odd) weights of channel slot 4 (G / 2) + c / 2.  Output
+// lane l then holds leaf l % 32's two channels of slots 4 (l / 32) + {0, 1}.
 double build_frags(const Shared& SH, int set, int L, int r, int N, const std::vector<double>& cs,
                    const std::vector<double>& sn) {
     const std::vector<int>& sp = SH.sets[set];
     const int mt = SH.mt(), nent = (int)sp.size() * r, nbs = std::max(1, (nent + 15) / 16);
+    const bool small = unc_small_last_tile(nbs, mt, SH.lr);
     uint4* out = SH.frags + SH.set_off[set];
     double rho = 0;
     for (int t = 0; t < mt; ++t)
@@ -831,17 +840,28 @@ double build_frags(const Shared& SH, int set, int L, int r, int N, const std::ve
             for (int lane = 0; lane < 64; ++lane) {
                 uint16_t hv[8];
                 for (int i8 = 0; i8 < 8; ++i8) {
-                    const int row = lane % 32;
-                    const int e = 16 * (st % nbs) + 8 * (lane / 32) + i8;
+                    int e, slot;
+                    bool im;
+                    if (small && t == mt - 1) {
+                        const int m = lane % 16, G = m / 4, c = m % 4, g = lane / 16;
+                        e = (G % 2 == g % 2) ? 8 * (g / 2) + i8 : nent;  // other half of the leaves: zero
+                        slot = 4 * (G / 2) + c / 2;
+                        im = (c % 2) != 0;
+                    } else {
+                        const int row = lane % 32;
+                        e = 16 * (st % nbs) + 8 * (lane / 32) + i8;
+                        slot = row % 16;
+                        im = row >= 16;
+                    }
                     const int part = st / nbs;
-                    const size_t qs = 16 * t + row % 16;
+                    const size_t qs = 16 * t + slot;
                     uint16_t v = 0;
                     if (e < nent && qs < SH.chan.size() && SH.chan[qs] > 0) {
                         const int k = SH.chan[qs];
                         const int j = sp[e / r], i = e % r;
                         const int p = j + L * i;
                         const int mm = (int)(((int64_t)k * p) % N);
-                        const double w = row >= 16 ? sn[mm] : cs[mm];
+                        const double w = im ? sn[mm] : cs[mm];
                         const uint16_t hi = bf16_rne_u((float)w);
                         const double r1 = w - bf16_d(hi);
                         const uint16_t lo = bf16_rne_u((float)r1);
