@@ -186,14 +186,20 @@ struct JoinCtl2 {
     unsigned long long out;       // emitted pairs
     unsigned long long checks;    // full PAF checks
     unsigned long long ncand;     // candidate pairs (equal hashes)
-    unsigned long long overflow;  // a join partition held more than kPartCap rows of one role
+    unsigned long long overflow;  // a join sub-bucket held more than kSubCap rows
 };
 
 // The batched join partitions the survivors by hash: P partitions (a power of
 // two, 64 <= P <= pmax, about 512 rows or fewer each on average), each with
 // kPartCap slots for A rows and kPartCap for B rows, records
 // (hash bits 0..31 << 32) | survivor index.  The partition is hash bits 40..
+// A partition's slots of one role are kPartSub sub-buckets of kSubCap, one per
+// XCD (blocks go to the 8 XCDs round-robin by block index): each sub-bucket's
+// cache lines are then written through one XCD's L2 only, and leave it whole
+// (appends from several XCDs to one line wrote back a partial line each).
 constexpr int kPartCap = 1024;
+constexpr int kPartSub = 8;
+constexpr int kSubCap = kPartCap / kPartSub;
 __host__ __device__ inline int64_t part_count(int64_t n, int64_t pmax) {
     int64_t P = 64;
     while (P < pmax && P * 512 < n) P <<= 1;
@@ -252,9 +258,10 @@ __global__ void k_unc_rows(const UncSide* __restrict__ sides, const int32_t* __r
         }
         const uint64_t hv = splitmix64(h);
         if (part) {
-            const uint64_t q = (uint64_t)(S.role == 1) * pmax + ((hv >> 40) & pmask);
+            const uint64_t q = ((uint64_t)(S.role == 1) * pmax + ((hv >> 40) & pmask)) * kPartSub +
+                               blockIdx.x % kPartSub;
             const uint32_t slot = atomicAdd(&pcnt[q], 1u);
-            if (slot < (uint32_t)kPartCap) part[q * kPartCap + slot] = (hv << 32) | (uint64_t)(uint32_t)i;
+            if (slot < (uint32_t)kSubCap) part[q * kSubCap + slot] = (hv << 32) | (uint64_t)(uint32_t)i;
             else jctl->overflow = 1ull;
         } else {
             // the row's role in bit 0, so that insert and probe read one array
@@ -272,8 +279,9 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
 // of 2 kPartCap slots (at most half full), its A records probe it; every B row
 // with the A row's 32-bit tag in the same partition is a candidate pair,
 // checked exactly by k_pair_check (tags that collide are dropped there).  A
-// block loads its partition's records (kPartCap / 256 of each role per thread)
-// before it clears the table, so the loads overlap the clearing.
+// block loads its partition's records (kPartCap / 256 slots of each role per
+// thread, slot i in sub-bucket i / kSubCap) before it clears the table, so the
+// loads overlap the clearing.
 __global__ void __launch_bounds__(256) k_part_join(const uint64_t* __restrict__ part, const uint32_t* __restrict__ pcnt,
                                                    int64_t pmax, const unsigned long long* dcount, int64_t dcap,
                                                    uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b,
@@ -283,15 +291,16 @@ __global__ void __launch_bounds__(256) k_part_join(const uint64_t* __restrict__ 
     __shared__ unsigned long long tab[T];
     const int64_t P = part_count(list_count(dcount, 0, dcap), pmax);
     for (int64_t p = blockIdx.x; p < P; p += gridDim.x) {
-        const uint32_t na = min(pcnt[p], (uint32_t)kPartCap), nb = min(pcnt[pmax + p], (uint32_t)kPartCap);
+        const uint32_t* ca = pcnt + p * kPartSub;
+        const uint32_t* cb = pcnt + (pmax + p) * kPartSub;
         const uint64_t* __restrict__ ra = part + p * kPartCap;
         const uint64_t* __restrict__ rb = part + (pmax + p) * kPartCap;
         unsigned long long ea[kPer], eb[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const uint32_t i = q * 256 + threadIdx.x;
-            ea[q] = i < na ? ra[i] : ~0ull;
-            eb[q] = i < nb ? rb[i] : ~0ull;
+            const uint32_t i = q * 256 + threadIdx.x, x = i / kSubCap, slot = i % kSubCap;
+            ea[q] = slot < ca[x] ? ra[i] : ~0ull;
+            eb[q] = slot < cb[x] ? rb[i] : ~0ull;
         }
         for (uint32_t e = threadIdx.x; e < T; e += blockDim.x) tab[e] = ~0ull;
         __syncthreads();
@@ -1685,7 +1694,7 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     while (pmax * 512 < F->rcap) pmax <<= 1;
     const int q_target = B.bound + paf_target;
     uint64_t* part = ctx->dev<uint64_t>("u_part" + sfx, (size_t)2 * pmax * kPartCap);
-    uint32_t* pcnt = ctx->dev<uint32_t>("u_pcnt" + sfx, (size_t)2 * pmax);
+    uint32_t* pcnt = ctx->dev<uint32_t>("u_pcnt" + sfx, (size_t)2 * pmax * kPartSub);
     F->jctl = ctx->dev<JoinCtl2>("u_jctl" + sfx, 1);
     F->ecap = ctx->unc_ecap;
     if (const char* e = std::getenv("GOLAY_UNC_EMIT_CAP"))  // fixed emitted-pair capacity (tests)
@@ -1717,8 +1726,8 @@ std::shared_ptr<UncInflight> unc_enqueue(gp_ctx* ctx, std::shared_ptr<UncPrep> p
     GP_HIP(hipGetLastError());
     const unsigned long long* nsurv = &F->ctl->surv;
     const unsigned grid = (unsigned)std::min<int64_t>((F->rcap + 255) / 256, (int64_t)ctx->num_cus * 16);
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<int64_t>((2 * pmax + 255) / 256, ctx->num_cus * 4)),
-                       dim3(256), 0, st, pcnt, 2 * pmax);
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<int64_t>((2 * pmax * kPartSub + 255) / 256, ctx->num_cus * 4)),
+                       dim3(256), 0, st, pcnt, 2 * pmax * kPartSub);
     if (B.all_pm1)
         hipLaunchKernelGGL(k_unc_rows<true>, dim3(grid), dim3(256), 0, st, B.d_sides, B.d_radix, B.d_tup, B.d_pm1,
                            d_surv, (int64_t)0, paf_target, q_target, nullptr, nsurv, F->rcap, F->jctl, part, pcnt, pmax);
