@@ -89,7 +89,9 @@ int gp_set_psd_mode(gp_ctx* ctx, int mode);
 /* Tuning table of the context (no counterpart in the reference): measured launch
  * and batch parameters by key, e.g. "unc_waves/50" (filter waves per SIMD at
  * finer length 50), "unc_leaf_budget/50" (leaves per uncompression batch),
- * "gen_ranks_per_lane/10/6" (gen ranks per lane at length 10, 6 letters).  A
+ * "unc_counter_streams/50" and "unc_counter_rounds/50" (how the filter hands out
+ * its work items), "gen_ranks_per_lane/10/6" (gen ranks per lane at length 10,
+ * 6 letters).  A
  * context starts with the table of the tuning file (GOLAY_TUNING, else
  * golaypair_amd/tuning.tsv, written by python -m golaypair_amd.autotune); every
  * setting gives the same results, only the time changes.  gp_set_tuning with a
