@@ -86,6 +86,20 @@ __device__ __forceinline__ float tile_max_psd(const f32x16 (&acc)[MT]) {
     return fmaxf(m, o);
 }
 
+// Maxima of two one-M-tile tiles in one lane-half swap: lane l < 32 returns
+// max_k PSD_k of candidate l of tile 0, lane l >= 32 that of candidate l - 32
+// of tile 1 (v_permlane32_swap(a, b) exchanges lanes 32..63 of a with lanes
+// 0..31 of b; |X|^2 >= 0, so maxima are taken on the float bit patterns).
+__device__ __forceinline__ float tile_pair_max_psd1(const f32x16& acc0, const f32x16& acc1) {
+    uint32_t m0 = 0u, m1 = 0u;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) m0 = max(m0, __float_as_uint(__builtin_fmaf(acc0[r], acc0[r], acc0[r + 8] * acc0[r + 8])));
+#pragma unroll
+    for (int r = 0; r < 8; ++r) m1 = max(m1, __float_as_uint(__builtin_fmaf(acc1[r], acc1[r], acc1[r + 8] * acc1[r + 8])));
+    const auto sw = __builtin_amdgcn_permlane32_swap(m0, m1, false, false);
+    return __uint_as_float(max((uint32_t)sw[0], (uint32_t)sw[1]));
+}
+
 // The k (1-based) held by register r (< 8) of M-tile t in lane `lane`.
 __device__ __forceinline__ int tile_k(int t, int r, int lane) {
     return 16 * t + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3) + 1;

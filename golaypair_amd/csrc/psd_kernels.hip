@@ -224,6 +224,46 @@ __global__ void __launch_bounds__(256) k_psd_mfma(const int8_t* __restrict__ wor
     uint2 ring[D][NB];
 #pragma unroll
     for (int g = 0; g < D; ++g) load(q0 + g, ring[g]);
+    if constexpr (MT == 1) {
+        // Tiles in pairs (q, q + 1): two independent MFMA chains, one lane-half
+        // swap for both maxima (tile q in lanes 0..31, tile q + 1 in lanes
+        // 32..63), so lane l decides row 32 q + l and one store covers 64 rows.
+        static_assert(D % 2 == 0, "tile pairs");
+        for (int64_t qq = q0; qq < q1; qq += D) {
+#pragma unroll
+            for (int g = 0; g < D; g += 2) {
+                const int64_t q = qq + g;
+                if (q >= q1) break;
+                uint4 b0[NB], b1[NB];
+#pragma unroll
+                for (int bl = 0; bl < NB; ++bl) {
+                    b0[bl] = pack8_bf16(ring[g][bl].x, ring[g][bl].y);
+                    b1[bl] = pack8_bf16(ring[g + 1][bl].x, ring[g + 1][bl].y);
+                }
+                load(q + D, ring[g]);
+                load(q + 1 + D, ring[g + 1]);
+                f32x16 acc0[1], acc1[1];
+                acc0[0] = f32x16{};
+                acc1[0] = f32x16{};
+#pragma unroll
+                for (int s2 = 0; s2 < 2 * NB; ++s2) {
+                    acc0[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[0][s2]), as_bf16x8(b0[s2 % NB]),
+                                                                     acc0[0], 0, 0, 0);
+                    acc1[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[0][s2]), as_bf16x8(b1[s2 % NB]),
+                                                                     acc1[0], 0, 0, 0);
+                }
+                const float m = tile_pair_max_psd1(acc0[0], acc1[0]);
+                const int64_t r = q * 32 + lane;
+                const bool mine = r < rows && (lane < 32 || q + 1 < q1);
+                const uint8_t dec = m > th.hi ? kReject : (m < th.lo ? kAccept : kBand);
+                if (mine) {
+                    decision[row_first + r] = dec;
+                    if (maxpsd) maxpsd[row_first + r] = m;
+                }
+            }
+        }
+        return;
+    }
     for (int64_t qq = q0; qq < q1; qq += D) {
 #pragma unroll
         for (int g = 0; g < D; ++g) {
