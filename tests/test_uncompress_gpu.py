@@ -170,6 +170,28 @@ def test_uncompress_side_mid_rows_beyond_one_chunk(gpu_ctx, seqflag):
     assert "".join(l + "\n" for l in cands.lines()) == ref_text
 
 
+@pytest.mark.parametrize("seq", [
+    [-3, -1, 1, -3, -1, -3, -3, -1, 1, 1, -1, -3, 1, -3, 1, -1, 1],              # N = 51: 17 channels
+    [-3, -1, 1, 1, -3, -1, -3, -3, 3, 1, -3, -1, -1, -1, -3, -1, -1, 1],         # N = 54: 18
+    [3, -1, -1, -1, -3, -1, -3, 1, -3, -3, -1, -3, -1, 1, -3, -3, -1, 1, -1],    # N = 57: 19
+    [-1, -1, 1, -1, -1, -3, -3, -3, 1, -3, -3, -3, -3, -1, 1, -1, -3, 1, -3, 1],  # N = 60: 20
+])
+@pytest.mark.parametrize("seqflag", [0, 1])
+def test_uncompress_side_small_last_tile(gpu_ctx, seq, seqflag):
+    """Sides expanded 3 -> 1 to lengths 51, 54, 57 and 60: 17 to 20 filter
+    channels (the k = 3m are fixed), so the second M-tile holds 1 to 4 of them
+    and runs on the 16x16x32 MFMAs of the small-tile class, padding slots
+    included (the v=50 level has 4).  Words of 11 letters +-1 (3 splits each)
+    and +-3 elsewhere, whose fixed channels pass the bound (found by a seeded
+    search); DFS candidate lines vs the oracle."""
+    seq = np.array(seq, np.int8)
+    ref_text, ref_leaves = _oracle.uncompress_side(seq, 3, 1, -2, seqflag)
+    cands, rot, st = gpu_ctx.uncompress_side(seq, 3, 1, -2, seqflag)
+    assert ref_leaves > 0 and st.psd_decisions == ref_leaves
+    assert len(ref_text) > 0
+    assert "".join(l + "\n" for l in cands.lines()) == ref_text
+
+
 def test_uncompress_side_v50_full_shape(gpu_ctx):
     """Single sequences of the v=50 5-compression level expanded 5 -> 1 (length 50:
     two M-tiles, fixed channels k = 5m dropped, single-split positions folded):
